@@ -1,0 +1,135 @@
+/*
+ * C ABI of the MI355X (gfx950) LU solver library (libops_solver_hip.so).
+ *
+ * Drop-in for the LU family of the reference's include/cann_ops_solver.h: the same entry
+ * point names, argument order, memory domains, layouts, 1-based pivots, info and status
+ * conventions. Each declaration cites the reference declaration it replaces. Entry points
+ * marked EXTENSION have no counterpart in the reference.
+ *
+ * Conventions (reference include/cann_ops_solver.h:82-99):
+ *  - getrf / getri / batched entry points take Device pointers; matrices are column-major
+ *    with element (r, c) at A[c * lda + r]; ipiv is 1-based; devInfo is a Device int.
+ *  - info = 0 on success, -i for an illegal argument i (handle excluded), or the first
+ *    1-based column with an exact zero on U's diagonal (the factorization still completes).
+ *  - Lwork counts are elements (float or complex), not bytes.
+ *  - aclsolverSgetrs / aclsolverCgetrs take HOST buffers in ROW-major layout and stage them
+ *    (reference :980-1043); the Device column-major solves are EXTENSIONS.
+ *  - Work is asynchronous on the handle's stream; validation steps that inspect Device data
+ *    synchronize that stream, as the reference does.
+ *  - Size limits are extended beyond the reference's (getrf m <= 32768, matinv n <= 64,
+ *    cgetrf m <= 16384); aclsolverSetStrictLimits(handle, 1) restores the reference limits.
+ */
+#pragma once
+#include "cann_ops_solver_common.h"
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+    /* ---- handle (reference include/cann_ops_solver.h:50-80) ---- */
+    aclsolverStatus_t aclsolverCreate(aclsolverHandle_t* handle);                        /* :50 */
+    aclsolverStatus_t aclsolverDestroy(aclsolverHandle_t handle);                        /* :61 */
+    aclsolverStatus_t aclsolverSetStream(aclsolverHandle_t handle, aclrtStream stream);  /* :69 */
+    aclsolverStatus_t aclsolverGetStream(aclsolverHandle_t handle, aclrtStream* stream); /* :79 */
+
+    /* EXTENSION: 1 = enforce the reference's size limits (m <= 8192 for sgetrf, n <= 32
+     * for matinv, m <= 4096 / n <= 8192 for cgetrf, n, nrhs <= 8192 for getrs). */
+    aclsolverStatus_t aclsolverSetStrictLimits(aclsolverHandle_t handle, int strict);
+
+    /* ---- real LU (reference include/cann_ops_solver.h:100-111) ---- */
+    aclsolverStatus_t aclsolverSgetrf_bufferSize(aclsolverHandle_t handle, int m, int n, float* A, int lda,
+                                                 int* Lwork); /* :100 */
+    aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n, float* A, int lda, float* Workspace,
+                                      int* devIpiv, int* devInfo); /* :101-102 */
+    aclsolverStatus_t aclsolverSgetri_bufferSize(aclsolverHandle_t handle, int n, float* A, int lda,
+                                                 int* Lwork); /* :103 */
+    aclsolverStatus_t aclsolverSgetri(aclsolverHandle_t handle, int n, float* A, int lda, int* devIpiv,
+                                      float* Workspace, int Lwork, int* devInfo); /* :104-105 */
+    aclsolverStatus_t aclsolverSgetriBatched(aclsolverHandle_t handle, int n, const float* const Aarray[], int lda,
+                                             const int* PivotArray, float* const Carray[], int ldc, int* infoArray,
+                                             int batchSize); /* :106-108 */
+    aclsolverStatus_t aclsolverSmatinvBatched(aclsolverHandle_t handle, int n, const float* const A[], int lda,
+                                              float* const Ainv[], int lda_inv, int* info,
+                                              int batchSize); /* :109-110 */
+
+    /* ---- complex LU (reference include/cann_ops_solver.h:882-923) ---- */
+    aclsolverStatus_t aclsolverCgetrf_bufferSize(aclsolverHandle_t handle, int m, int n, aclFloatComplex* A, int lda,
+                                                 int* Lwork); /* :882-883 */
+    aclsolverStatus_t aclsolverCgetrf(aclsolverHandle_t handle, int m, int n, aclFloatComplex* A, int lda,
+                                      aclFloatComplex* Workspace, int* devIpiv, int* devInfo); /* :890-891 */
+    aclsolverStatus_t aclsolverCgetri_bufferSize(aclsolverHandle_t handle, int n, aclFloatComplex* A, int lda,
+                                                 int* Lwork); /* :896-897 */
+    aclsolverStatus_t aclsolverCgetri(aclsolverHandle_t handle, int n, aclFloatComplex* A, int lda, int* devIpiv,
+                                      aclFloatComplex* Workspace, int Lwork, int* devInfo); /* :903-904 */
+    aclsolverStatus_t aclsolverCgetriBatched(aclsolverHandle_t handle, int n, const aclFloatComplex* const Aarray[],
+                                             int lda, const int* PivotArray, aclFloatComplex* const Carray[], int ldc,
+                                             int* infoArray, int batchSize); /* :911-913 */
+    aclsolverStatus_t aclsolverCmatinvBatched(aclsolverHandle_t handle, int n, const aclFloatComplex* const A[],
+                                              int lda, aclFloatComplex* const Ainv[], int lda_inv, int* info,
+                                              int batchSize); /* :919-921 */
+
+    /* ---- host-staged row-major solves (reference include/cann_ops_solver.h:1011-1043) ---- */
+    aclError aclsolverSgetrs(aclsolverHandle_t handle, const uint64_t trans, const int64_t n, const int64_t nrhs,
+                             float* A, const int64_t lda, int32_t* ipiv, float* B, const int64_t ldb,
+                             int32_t* info); /* :1011-1012 */
+    aclError aclsolverCgetrs(aclsolverHandle_t handle, const uint64_t trans, const int64_t n, const int64_t nrhs,
+                             aclFloatComplex* A, const int64_t lda, int32_t* ipiv, aclFloatComplex* B,
+                             const int64_t ldb, int32_t* info); /* :1040-1042 */
+
+    /* ---- EXTENSION: Device column-major solves from getrf factors ----
+     * trans 0: A X = B; 1: A^T X = B; 2 (complex only): A^H X = B. B is overwritten by X.
+     * devIpiv may be null (factors without pivoting). devInfo receives 0 or -i. */
+    aclsolverStatus_t aclsolverSgetrsDevice(aclsolverHandle_t handle, int trans, int n, int nrhs, const float* dA,
+                                            int lda, const int* devIpiv, float* dB, int ldb, int* devInfo);
+    aclsolverStatus_t aclsolverCgetrsDevice(aclsolverHandle_t handle, int trans, int n, int nrhs,
+                                            const aclFloatComplex* dA, int lda, const int* devIpiv,
+                                            aclFloatComplex* dB, int ldb, int* devInfo);
+
+    /* ---- EXTENSION: mixed-precision dense solve with GMRES iterative refinement ----
+     * Argument convention of the reference's aclsolverSSgels (include/cann_ops_solver.h:
+     * 815-862, the cuSOLVER IRS form). A is FP32 column-major and is NOT modified; the
+     * factors live in dWorkspace. The trailing-matrix updates of the LU run on MFMA with
+     * FP16 (SH, per-panel power-of-two scaling) or BF16 (SB) inputs and FP32 accumulation;
+     * SS factors in plain FP32. Refinement: GMRES preconditioned by the LU factors, the
+     * residual r = b - A x and the correction accumulation in FP64, until the backward
+     * error ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= the handle's tolerance
+     * (default 1e-6) or maxiter outer steps. niter (host) receives the total number of
+     * GMRES iterations (negative: refinement did not converge and the FP32 LU solution
+     * was returned). dinfo (device) gets 0, -i or the first zero pivot. */
+    aclsolverStatus_t aclsolverSHgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes);
+    aclsolverStatus_t aclsolverSBgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes);
+    aclsolverStatus_t aclsolverSSgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes);
+    aclsolverStatus_t aclsolverSHgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                      float* dB, int lddb, float* dX, int lddx, void* dWorkspace, size_t lwork_bytes,
+                                      int* niter, int* dinfo);
+    aclsolverStatus_t aclsolverSBgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                      float* dB, int lddb, float* dX, int lddx, void* dWorkspace, size_t lwork_bytes,
+                                      int* niter, int* dinfo);
+    aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                      float* dB, int lddb, float* dX, int lddx, void* dWorkspace, size_t lwork_bytes,
+                                      int* niter, int* dinfo);
+    /* EXTENSION: refinement controls for the gesv family (tolerance on the backward error,
+     * maximum outer refinement steps, GMRES restart length). */
+    aclsolverStatus_t aclsolverSetRefinement(aclsolverHandle_t handle, double tolerance, int maxOuter, int restart);
+
+    /* ---- EXTENSION: uneven batched LU solve (getrf + getrs) in one persistent launch ----
+     * nArray (HOST, batchSize entries) holds each order n_i; dAarray / dBarray are Device
+     * arrays of Device pointers to column-major A_i (n_i x n_i, lda = n_i) and B_i
+     * (n_i x nrhs, ldb = n_i); A_i is overwritten by its packed LU, B_i by X_i. dIpiv holds
+     * sum(n_i) pivots, matrix i's at offset sum_{j<i} n_j. dInfo gets one info per matrix.
+     * A plan bins the matrices by size and holds the tuned per-bin schedule; executing a
+     * plan allocates nothing, synchronizes nothing and may be captured into a hipGraph. */
+    typedef void* aclsolverVbatchedPlan_t;
+    aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t handle, int batchSize, const int* nArray,
+                                                        int nrhs, aclsolverVbatchedPlan_t* plan);
+    aclsolverStatus_t aclsolverSgesvVbatched_execute(aclsolverHandle_t handle, aclsolverVbatchedPlan_t plan,
+                                                     float* const dAarray[], float* const dBarray[], int* dIpiv,
+                                                     int* dInfo);
+    aclsolverStatus_t aclsolverSgesvVbatched_destroyPlan(aclsolverVbatchedPlan_t plan);
+    aclsolverStatus_t aclsolverSgesvVbatched(aclsolverHandle_t handle, int batchSize, const int* nArray, int nrhs,
+                                             float* const dAarray[], float* const dBarray[], int* dIpiv, int* dInfo);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,423 @@
+// MFMA GEMM update C -= op(A) * B and the LDS-resident diagonal-block triangular solve.
+//
+// The GEMM is the trailing-matrix update of the blocked LU (reference
+// src/sgetrf/real_lu_blocked_kernel.cpp:1119-1158, real_lu_factor_product_kernel: FP32 with
+// HF32 off, product rounded once before it is added). Here it runs on gfx950 matrix cores:
+//   FP32: v_mfma_f32_32x32x2_f32, BF16: v_mfma_f32_32x32x16_bf16, FP16: v_mfma_f32_32x32x16_f16,
+// always with FP32 accumulation. A workgroup of 4 wave64s owns a 128x128 tile of C; each
+// wave owns a 64x64 quadrant (2x2 blocks of 32x32 accumulators, 64 accumulator registers).
+// Operand tiles are staged through LDS; the FP16 variant scales each staged operand by a
+// power of two chosen from the tile's largest magnitude (exact, undone after the sum).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "lu_kernels.h"
+
+namespace ops
+{
+namespace
+{
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int TM = 128;
+constexpr int TN = 128;
+
+// Maps the linear workgroup id onto (tile row, tile column). Consecutive workgroups are
+// dealt round-robin to the 8 XCDs; regroup them so that each XCD gets a contiguous run of
+// tiles (column-major tile order), which then share operand panels in that XCD's L2.
+__device__ inline void TileOf(int tilesM, int tilesN, int* tm, int* tn)
+{
+    int total = tilesM * tilesN;
+    int id = blockIdx.x;
+    if (total % 8 == 0)
+    {
+        int per = total / 8;
+        id = (id % 8) * per + id / 8;
+    }
+    *tm = id % tilesM;
+    *tn = id / tilesM;
+}
+
+__device__ inline float OpA(const float* A, int lda, int transA, int r, int k)
+{
+    return transA ? A[static_cast<size_t>(r) * lda + k] : A[static_cast<size_t>(k) * lda + r];
+}
+
+template <int TRANSA>
+__global__ __launch_bounds__(256) void gemm_sub_f32_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                           const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                           int ldc, int tilesM, int tilesN)
+{
+    __shared__ float As[16][TM + 4];
+    __shared__ float Bs[16][TN + 4];
+    int tm, tn;
+    TileOf(tilesM, tilesN, &tm, &tn);
+    const int row0 = tm * TM, col0 = tn * TN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    for (int k0 = 0; k0 < K; k0 += 16)
+    {
+        if (!TRANSA)
+        {
+            const int r = tid & 127, kq = (tid >> 7) * 8;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+            {
+                int k = kq + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(k0 + k) * lda + row0 + r];
+                As[k][r] = v;
+            }
+        }
+        else
+        {
+            const int r = tid >> 1, kh = (tid & 1) * 8;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+            {
+                int k = kh + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(row0 + r) * lda + k0 + k];
+                As[k][r] = v;
+            }
+        }
+        {
+            const int j = tid >> 1, kh = (tid & 1) * 8;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+            {
+                int k = kh + kk;
+                float v = 0.0f;
+                if (col0 + j < N && k0 + k < K) v = B[static_cast<size_t>(col0 + j) * ldb + k0 + k];
+                Bs[k][j] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 2)
+        {
+            const int k = kk + (lane >> 5);
+            const float a0 = As[k][wm * 64 + (lane & 31)];
+            const float a1 = As[k][wm * 64 + 32 + (lane & 31)];
+            const float b0 = Bs[k][wn * 64 + (lane & 31)];
+            const float b1 = Bs[k][wn * 64 + 32 + (lane & 31)];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // 32x32 accumulator layout: register v of lane l holds row (v%4) + 8*(v/4) + 4*(l/32),
+    // column l%32 of the block.
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+        {
+            const int j = col0 + wn * 64 + bj * 32 + (lane & 31);
+            if (j >= N) continue;
+            float* cj = C + static_cast<size_t>(j) * ldc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+            {
+                const int i = row0 + wm * 64 + bi * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                if (i < M) cj[i] -= acc[bi][bj][v];
+            }
+        }
+}
+
+template <typename T>
+__device__ inline T ToHalf(float v);
+template <>
+__device__ inline _Float16 ToHalf<_Float16>(float v)
+{
+    return static_cast<_Float16>(v);
+}
+template <>
+__device__ inline __bf16 ToHalf<__bf16>(float v)
+{
+    return static_cast<__bf16>(v);
+}
+
+__device__ inline float BlockMax(float v, float* red)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) r = fmaxf(r, red[w]);
+    return r;
+}
+
+// Power-of-two factor that brings the largest magnitude into [2^14, 2^15) (FP16 keeps
+// 2^15 < 65504); clamped so that the factor and its inverse stay finite normal floats.
+__device__ inline float ScaleFor(float maxAbs)
+{
+    if (!(maxAbs > 0.0f) || !(maxAbs <= FLT_MAX)) return 1.0f;
+    int e = ilogbf(maxAbs);
+    int s = 14 - e;
+    s = s > 100 ? 100 : (s < -100 ? -100 : s);
+    return ldexpf(1.0f, s);
+}
+
+template <typename HT, int TRANSA, bool SCALE>
+__global__ __launch_bounds__(256) void gemm_sub_half_kernel(int M, int N, int K, const float* __restrict__ A,
+                                                            int lda, const float* __restrict__ B, int ldb,
+                                                            float* __restrict__ C, int ldc, int tilesM, int tilesN)
+{
+    typedef HT hx8 __attribute__((ext_vector_type(8)));
+    __shared__ __attribute__((aligned(16))) HT As[TM][32 + 8];
+    __shared__ __attribute__((aligned(16))) HT Bs[TN][32 + 8];
+    __shared__ float red[8];
+    int tm, tn;
+    TileOf(tilesM, tilesN, &tm, &tn);
+    const int row0 = tm * TM, col0 = tn * TN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    float sa = 1.0f, sb = 1.0f;
+    if (SCALE)
+    {
+        float ma = 0.0f, mb = 0.0f;
+        for (int e = tid; e < TM * K; e += 256)
+        {
+            int r = TRANSA ? e / K : e % TM, k = TRANSA ? e % K : e / TM;
+            if (row0 + r < M) ma = fmaxf(ma, fabsf(OpA(A, lda, TRANSA, row0 + r, k)));
+        }
+        for (int e = tid; e < TN * K; e += 256)
+        {
+            int j = e / K, k = e % K;
+            if (col0 + j < N) mb = fmaxf(mb, fabsf(B[static_cast<size_t>(col0 + j) * ldb + k]));
+        }
+        sa = ScaleFor(BlockMax(ma, red));
+        sb = ScaleFor(BlockMax(mb, red));
+    }
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    for (int k0 = 0; k0 < K; k0 += 32)
+    {
+        if (!TRANSA)
+        {
+            const int r = tid & 127, kq = (tid >> 7) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                int k = kq + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(k0 + k) * lda + row0 + r];
+                As[r][k] = ToHalf<HT>(v * sa);
+            }
+        }
+        else
+        {
+            const int r = tid >> 1, kh = (tid & 1) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                int k = kh + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(row0 + r) * lda + k0 + k];
+                As[r][k] = ToHalf<HT>(v * sa);
+            }
+        }
+        {
+            const int j = tid >> 1, kh = (tid & 1) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                int k = kh + kk;
+                float v = 0.0f;
+                if (col0 + j < N && k0 + k < K) v = B[static_cast<size_t>(col0 + j) * ldb + k0 + k];
+                Bs[j][k] = ToHalf<HT>(v * sb);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 16)
+        {
+            const int kq = ks + (lane >> 5) * 8;
+            const hx8 a0 = *reinterpret_cast<const hx8*>(&As[wm * 64 + (lane & 31)][kq]);
+            const hx8 a1 = *reinterpret_cast<const hx8*>(&As[wm * 64 + 32 + (lane & 31)][kq]);
+            const hx8 b0 = *reinterpret_cast<const hx8*>(&Bs[wn * 64 + (lane & 31)][kq]);
+            const hx8 b1 = *reinterpret_cast<const hx8*>(&Bs[wn * 64 + 32 + (lane & 31)][kq]);
+            if constexpr (sizeof(HT) == 2 && __is_same(HT, _Float16))
+            {
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, acc[1][1], 0, 0, 0);
+            }
+            else
+            {
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const float ia = 1.0f / sa, ib = 1.0f / sb;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+        {
+            const int j = col0 + wn * 64 + bj * 32 + (lane & 31);
+            if (j >= N) continue;
+            float* cj = C + static_cast<size_t>(j) * ldc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+            {
+                const int i = row0 + wm * 64 + bi * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                if (i < M) cj[i] -= (acc[bi][bj][v] * ia) * ib;
+            }
+        }
+}
+
+// B <- op(T)^{-1} B for a K x K triangle. One workgroup owns CT columns of B in LDS; the
+// row that step s completes is divided by its diagonal by the thread that applied the
+// last update to it, so each step needs a single barrier.
+template <int CT, bool LOWER, bool UNIT, bool TRANS>
+__global__ __launch_bounds__(256) void trsm_block_kernel(int K, int N, const float* __restrict__ T, int ldt,
+                                                         float* __restrict__ B, int ldb)
+{
+    extern __shared__ float sB[];  // K x (CT + 1)
+    constexpr int G = 256 / CT;    // row groups
+    constexpr bool FWD = LOWER != TRANS;
+    const int tid = threadIdx.x, c = tid % CT, g = tid / CT;
+    const int colBase = blockIdx.x * CT;
+    const bool active = colBase + c < N;
+    float* bcol = B + static_cast<size_t>(colBase + c) * ldb;
+    auto opT = [&](int i, int s) -> float
+    { return TRANS ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i]; };
+    for (int i = g; i < K; i += G) sB[i * (CT + 1) + c] = active ? bcol[i] : 0.0f;
+    __syncthreads();
+    const int first = FWD ? 0 : K - 1;
+    if (!UNIT && g == 0 && active) sB[first * (CT + 1) + c] /= T[static_cast<size_t>(first) * ldt + first];
+    __syncthreads();
+    if (FWD)
+    {
+        for (int s = 0; s < K - 1; ++s)
+        {
+            const float x = sB[s * (CT + 1) + c];
+            for (int i = s + 1 + g; i < K; i += G)
+            {
+                float v = fmaf(-opT(i, s), x, sB[i * (CT + 1) + c]);
+                if (!UNIT && i == s + 1) v = v / T[static_cast<size_t>(i) * ldt + i];
+                sB[i * (CT + 1) + c] = v;
+            }
+            __syncthreads();
+        }
+    }
+    else
+    {
+        for (int s = K - 1; s > 0; --s)
+        {
+            const float x = sB[s * (CT + 1) + c];
+            for (int i = s - 1 - g; i >= 0; i -= G)
+            {
+                float v = fmaf(-opT(i, s), x, sB[i * (CT + 1) + c]);
+                if (!UNIT && i == s - 1) v = v / T[static_cast<size_t>(i) * ldt + i];
+                sB[i * (CT + 1) + c] = v;
+            }
+            __syncthreads();
+        }
+    }
+    if (active)
+        for (int i = g; i < K; i += G) bcol[i] = sB[i * (CT + 1) + c];
+}
+
+template <int CT, bool LOWER, bool UNIT, bool TRANS>
+hipError_t LaunchTrsm(int K, int N, const float* T, int ldt, float* B, int ldb, hipStream_t stream)
+{
+    size_t shmem = static_cast<size_t>(K) * (CT + 1) * sizeof(float);
+    int blocks = (N + CT - 1) / CT;
+    hipLaunchKernelGGL((trsm_block_kernel<CT, LOWER, UNIT, TRANS>), dim3(blocks), dim3(256), shmem, stream, K, N, T,
+                       ldt, B, ldb);
+    return hipGetLastError();
+}
+
+template <int CT>
+hipError_t TrsmDispatch(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
+                        hipStream_t stream)
+{
+    if (lower && unit && !transT) return LaunchTrsm<CT, true, true, false>(K, N, T, ldt, B, ldb, stream);
+    if (lower && unit && transT) return LaunchTrsm<CT, true, true, true>(K, N, T, ldt, B, ldb, stream);
+    if (lower && !unit && !transT) return LaunchTrsm<CT, true, false, false>(K, N, T, ldt, B, ldb, stream);
+    if (lower && !unit && transT) return LaunchTrsm<CT, true, false, true>(K, N, T, ldt, B, ldb, stream);
+    if (!lower && unit && !transT) return LaunchTrsm<CT, false, true, false>(K, N, T, ldt, B, ldb, stream);
+    if (!lower && unit && transT) return LaunchTrsm<CT, false, true, true>(K, N, T, ldt, B, ldb, stream);
+    if (!lower && !unit && !transT) return LaunchTrsm<CT, false, false, false>(K, N, T, ldt, B, ldb, stream);
+    return LaunchTrsm<CT, false, false, true>(K, N, T, ldt, B, ldb, stream);
+}
+
+}  // namespace
+
+hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, const float* B, int ldb, float* C,
+                   int ldc, Precision prec, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    const int tilesM = (M + TM - 1) / TM, tilesN = (N + TN - 1) / TN;
+    const int blocks = tilesM * tilesN;
+    if (prec == Precision::FP32)
+    {
+        if (transA)
+            hipLaunchKernelGGL(gemm_sub_f32_kernel<1>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb, C,
+                               ldc, tilesM, tilesN);
+        else
+            hipLaunchKernelGGL(gemm_sub_f32_kernel<0>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb, C,
+                               ldc, tilesM, tilesN);
+    }
+    else if (prec == Precision::FP16)
+    {
+        if (transA)
+            hipLaunchKernelGGL((gemm_sub_half_kernel<_Float16, 1, true>), dim3(blocks), dim3(256), 0, stream, M, N, K,
+                               A, lda, B, ldb, C, ldc, tilesM, tilesN);
+        else
+            hipLaunchKernelGGL((gemm_sub_half_kernel<_Float16, 0, true>), dim3(blocks), dim3(256), 0, stream, M, N, K,
+                               A, lda, B, ldb, C, ldc, tilesM, tilesN);
+    }
+    else
+    {
+        if (transA)
+            hipLaunchKernelGGL((gemm_sub_half_kernel<__bf16, 1, false>), dim3(blocks), dim3(256), 0, stream, M, N, K,
+                               A, lda, B, ldb, C, ldc, tilesM, tilesN);
+        else
+            hipLaunchKernelGGL((gemm_sub_half_kernel<__bf16, 0, false>), dim3(blocks), dim3(256), 0, stream, M, N, K,
+                               A, lda, B, ldb, C, ldc, tilesM, tilesN);
+    }
+    return hipGetLastError();
+}
+
+hipError_t TrsmBlock(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
+                     hipStream_t stream)
+{
+    if (K <= 0 || N <= 0) return hipSuccess;
+    if (K > 512) return hipErrorInvalidValue;
+    if (K <= 256) return TrsmDispatch<32>(K, N, T, ldt, lower, unit, transT, B, ldb, stream);
+    return TrsmDispatch<16>(K, N, T, ldt, lower, unit, transT, B, ldb, stream);
+}
+
+}  // namespace ops

@@ -1,0 +1,84 @@
+// Internal declarations shared by the host and kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "cann_ops_solver.h"
+
+#define OPS_API extern "C" __attribute__((visibility("default")))
+
+namespace ops
+{
+
+// Solver handle: the stream plus per-handle options and a reusable scratch allocation.
+// Reference: src/utils/aclsolver_handle_internal.h:33-100 (stream + per-context LU scratch).
+struct Handle
+{
+    uint32_t magic = 0x4C55AC01u;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int strict = 0;           // enforce the reference's size limits
+    double irTolerance = 1e-6;
+    int irMaxOuter = 30;
+    int irRestart = 30;
+    void* scratch = nullptr;  // handle-owned Device scratch (grown on demand)
+    int* dflag = nullptr;     // Device flag word for validation scans
+    int* hflag = nullptr;     // pinned host mirror of dflag
+    size_t scratchBytes = 0;
+    int numCUs = 256;
+};
+
+inline Handle* H(aclsolverHandle_t h) { return static_cast<Handle*>(h); }
+inline hipStream_t Stream(aclsolverHandle_t h) { return H(h)->stream; }
+
+// True when pointer is Device memory of the current process (hipMalloc / managed).
+bool IsDevice(const void* p);
+// True when pointer is NOT Device memory (plain host or pinned host).
+bool IsHostOnly(const void* p);
+
+// Grows the handle scratch to at least bytes; returns nullptr on failure. The scratch is
+// only touched by work on the handle's stream, so stream order protects reuse.
+void* HandleScratch(aclsolverHandle_t h, size_t bytes);
+
+// Writes value into count ints of a Device info array on stream.
+void FillInfo(int* info, int count, int value, hipStream_t stream);
+// Invalid argument: writes -argument into info (when it is a Device pointer) and returns
+// INVALID_VALUE (reference src/utils/real_lu_runtime.h:61-78).
+aclsolverStatus_t Invalid(aclsolverHandle_t h, int* info, int count, int argument);
+// Returns EXECUTION_FAILED when the last launch on this thread failed.
+aclsolverStatus_t Launched();
+// Scans a Device matrix (or batch of them) for NaN/Inf; returns true when all finite.
+// Synchronizes the stream (reference src/utils/real_lu_common.h:90-132).
+bool AllFinite(aclsolverHandle_t h, const float* a, int rows, int cols, int lda, int64_t batchStride, int batch,
+               bool* ok);
+bool AllFiniteArrays(aclsolverHandle_t h, const float* const* arrays, int rows, int cols, int lda, int batch,
+                     bool* ok);
+
+// ---- real LU engine (lu_engine.hip) ----
+enum class Precision
+{
+    FP32 = 0,
+    FP16 = 1,
+    BF16 = 2
+};
+// Workspace bytes needed by RealGetrf for an m x n matrix.
+size_t RealGetrfWorkspaceBytes(int m, int n);
+// Blocked right-looking LU with partial pivoting (or none when ipiv == nullptr).
+// Writes ipiv (1-based) and info (first zero pivot). Returns hipSuccess on launch success.
+hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
+                     hipStream_t stream);
+// Device column-major getrs from RealGetrf factors (trans 0 / 1). work: RealGetrsWorkspaceBytes.
+size_t RealGetrsWorkspaceBytes(int n, int nrhs);
+hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const int* ipiv, float* b, int ldb,
+                     void* work, hipStream_t stream);
+
+// ---- complex LU engine (complex_engine.hip) ----
+size_t ComplexGetrfWorkspaceBytes(int m, int n);
+hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream);
+size_t ComplexGetrsWorkspaceBytes(int n, int nrhs);
+hipError_t ComplexGetrs(int trans, int n, int nrhs, const float2* lu, int lda, const int* ipiv, float2* b, int ldb,
+                        void* work, hipStream_t stream);
+
+}  // namespace ops

@@ -1,0 +1,23 @@
+// Device building blocks of the real LU path, shared by getrf, getrs and the gesv family.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace ops
+{
+
+// C(MxN) -= op(A)(MxK) * B(KxN), column-major, on MFMA with FP32 accumulation. op(A) = A
+// (transA = 0) or A^T (transA = 1). prec selects the MFMA input type (FP32, or FP16 with
+// per-tile power-of-two scaling, or BF16). The product is accumulated from zero and
+// subtracted from C once (reference src/sgetrf/real_lu_blocked_kernel.cpp:1152-1153).
+hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, const float* B, int ldb, float* C,
+                   int ldc, Precision prec, hipStream_t stream);
+
+// Triangular solve with a K x K diagonal block T (column-major, ldt) on a K x N block B:
+// B <- op(T)^{-1} B. lower: T's lower triangle is used (else upper); unit: unit diagonal;
+// transT: op(T) = T^T. K <= 512.
+hipError_t TrsmBlock(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
+                     hipStream_t stream);
+
+}  // namespace ops
