@@ -1,0 +1,261 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X LU solver library.
+
+Metric: LU solve GFLOP/s at n=16384, BF16 MFMA factorization + GMRES-IR (FP64 residual)
+to a backward error <= 1e-6, beside the plain FP32 getrf+getrs on the same GPU.
+
+Workload (one step = one complete solve A x = b from an FP32 A already resident in HBM):
+  A = Q1 diag(s) Q2^T, n = 16384, Q1/Q2 from Householder QR of seeded N(0,1) matrices,
+  s log-spaced from 1 to 1e6 (cond 1e6); b = A x_true in FP64 (rounded to FP32),
+  x_true ~ N(0,1). value = (2/3 n^3 + 2 n^2) / wall time of the step (nominal LAPACK
+  count; refinement work is not credited).
+
+Also reported: the FP32 solve (SSgesv) time and ratio, backward/forward errors, the
+trailing-GEMM roofline (per-launch HIP events inside the timed region), the host LAPACK
+baseline on a bounded sample, and the batched-inverse configuration (65536 x 32x32).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "ops-solver_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "LU solve GFLOP/s n=16384 BF16+IR vs FP32 at 1e-6 bwd err; batched solves/s"
+PEAK_BF16 = 2.5e15  # dense MFMA, FLOP/s
+PEAK_HBM = 8.0e12  # B/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_system(torch, n, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    G1 = torch.randn(n, n, device="cuda", generator=g)
+    Q1, _ = torch.linalg.qr(G1)
+    del G1
+    G2 = torch.randn(n, n, device="cuda", generator=g)
+    Q2, _ = torch.linalg.qr(G2)
+    del G2
+    s = torch.logspace(0, 6, n, device="cuda", dtype=torch.float32)
+    a = (Q1 * s) @ Q2.T  # row-major A
+    del Q1, Q2
+    at = a.T.contiguous()  # row-major A^T == column-major A
+    del a
+    x_true = torch.randn(n, device="cuda", generator=g, dtype=torch.float64)
+    b64 = at.T.double() @ x_true
+    b = b64.float().contiguous()
+    return at, b, x_true
+
+
+def errors(torch, at, x, b, x_true):
+    a64 = at.T.double()
+    x64 = x.double()
+    r = b.double() - a64 @ x64
+    anorm = a64.abs().sum(dim=1).max()
+    berr = (r.abs().max() / (anorm * x64.abs().max() + b.double().abs().max())).item()
+    ferr = ((x64 - x_true).abs().max() / x_true.abs().max()).item()
+    del a64
+    return berr, ferr
+
+
+def gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo):
+    niter = ctypes.c_int(0)
+    f = getattr(A.lib(), f"aclsolver{name}gesv")
+    st = f(h.h, n, 1, A.ptr(at), n, A.ptr(dipiv), A.ptr(b), n, A.ptr(x), n, A.ptr(work), lwork,
+           ctypes.byref(niter), A.ptr(dinfo))
+    if st != A.SUCCESS:
+        raise RuntimeError(f"{name}gesv status {st}")
+    return niter.value
+
+
+def time_mode(A, torch, h, name, steps, warmup, n, at, b, x, work, lwork, dipiv, dinfo, profile=False):
+    iters = 0
+    for _ in range(warmup):
+        gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo)
+    torch.cuda.synchronize()
+    if profile:
+        A.lib().aclsolverProfileBegin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        iters = gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = None
+    if profile:
+        launches, work_f, ms = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+        A.lib().aclsolverProfileEnd(0, ctypes.byref(launches), ctypes.byref(work_f), ctypes.byref(ms))
+        prof = (launches.value, work_f.value, ms.value)
+    info = int(dinfo.item())
+    return (t1 - t0) * 1e3 / steps, iters, info, prof
+
+
+def cpu_baseline(n_cpu):
+    """Host LAPACK sgetrf+sgetrs (SciPy/OpenBLAS, the CPU path of the reference's tests)
+    on a bounded sample: one n_cpu system, median of 3 after 1 warm-up."""
+    import numpy as np
+    import scipy.linalg.lapack as la
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(4)
+    a = np.asfortranarray(rng.standard_normal((n_cpu, n_cpu)).astype(np.float32) / np.sqrt(n_cpu))
+    b = rng.standard_normal((n_cpu, 1)).astype(np.float32)
+    times = []
+    for rep in range(4):
+        t0 = time.perf_counter()
+        lu, piv, info = la.sgetrf(a)
+        x, info2 = la.sgetrs(lu, piv, b)
+        t = time.perf_counter() - t0
+        if rep:
+            times.append(t)
+    t = sorted(times)[len(times) // 2]
+    flops = 2.0 / 3.0 * n_cpu ** 3 + 2.0 * n_cpu ** 2
+    return {"value": round(flops / t / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads), "kind": "reference",
+            "sample": f"host LAPACK (SciPy OpenBLAS) sgetrf+sgetrs, one N(0,1)/sqrt(n) system of n={n_cpu}, "
+                      f"median of 3 runs ({t:.2f} s each) on {threads} threads"}
+
+
+def batched_inverse(A, torch, h, steps=20):
+    """65536 FP32 32x32 matrices, A_i = G_i/sqrt(32) + 2I: direct batched inverse."""
+    n, batch = 32, 65536
+    g = torch.Generator(device="cuda")
+    g.manual_seed(32)
+    a = torch.randn(batch, n, n, device="cuda", generator=g) / n ** 0.5 + 2 * torch.eye(n, device="cuda")
+    a = a.contiguous()
+    c = torch.empty_like(a)
+    ipiv = torch.empty(batch, n, dtype=torch.int32, device="cuda")
+    info = torch.empty(batch, dtype=torch.int32, device="cuda")
+    L = A.lib()
+
+    def run():
+        st = L.aclsolverSmatinvStridedBatched(h.h, n, A.ptr(a), n, n * n, A.ptr(c), n, n * n, A.ptr(ipiv),
+                                              A.ptr(info), batch)
+        assert st == A.SUCCESS
+
+    run()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        run()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    # check a sample against A C = I
+    idx = torch.arange(0, batch, 4093, device="cuda")
+    res = (torch.bmm(a[idx].transpose(1, 2).contiguous().double(), c[idx].transpose(1, 2).contiguous().double())
+           - torch.eye(n, device="cuda", dtype=torch.float64)).abs().max().item()
+    bytes_ = batch * (2 * n * n * 4 + n * 4 + 4)
+    return {"workload": "smatinv 65536 x 32x32 FP32, A_i = G_i/sqrt(32) + 2I (output inverse + ipiv + info)",
+            "ms": round(ms, 4), "matrices_per_s": round(batch / ms * 1e3, 1),
+            "GB_per_s": round(bytes_ / ms / 1e6, 1), "hbm_frac": round(bytes_ / ms / 1e-3 / PEAK_HBM, 4),
+            "max_abs_AC_minus_I_sample": res, "all_info_zero": bool((info == 0).all().item())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (block-cyclic / batched sharding) "
+                                   "path is a design only, --gpus must be 1"}))
+        return 2
+    import torch
+
+    import aclsolver as A
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = args.n
+    log(f"generating A = Q1 diag(s) Q2^T, n={n}")
+    at, b, x_true = make_system(torch, n, seed=16384)
+    torch.cuda.synchronize()
+    h = A.Handle()
+    A.lib().aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 30, 30)
+    lw = ctypes.c_size_t(0)
+    assert A.lib().aclsolverSBgesv_bufferSize(h.h, n, 1, ctypes.byref(lw)) == A.SUCCESS
+    work = torch.empty(lw.value, dtype=torch.uint8, device="cuda")
+    x = torch.empty(n, device="cuda")
+    dipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    dinfo = torch.empty(1, dtype=torch.int32, device="cuda")
+
+    res = {}
+    for name in ("SS", "SH", "SB"):
+        log(f"timing {name}gesv: warmup {args.warmup}, steps {args.steps}")
+        ms, iters, info, prof = time_mode(A, torch, h, name, args.steps, args.warmup, n, at, b, x, work, lw.value,
+                                          dipiv, dinfo, profile=(name == "SB"))
+        berr, ferr = errors(torch, at, x, b, x_true)
+        res[name] = {"ms": ms, "gmres_iterations": iters, "info": info, "backward_error": berr,
+                     "forward_error": ferr, "profile": prof}
+        log(f"{name}: {ms:.2f} ms, iters {iters}, info {info}, berr {berr:.3e}, ferr {ferr:.3e}")
+
+    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
+    sb = res["SB"]
+    value = flops / (sb["ms"] * 1e-3) / 1e9
+    launches, gflop, gms = sb["profile"]
+    achieved = gflop / (gms * 1e-3) if gms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sb["ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"dense solve n={n}, A=Q1 diag(s) Q2^T (s log-spaced 1..1e6, cond 1e6), 1 RHS; "
+                               "BF16 MFMA LU (FP32 accumulate) + GMRES-IR with FP64 residual to bwd err 1e-6",
+                   "n": n, "nrhs": 1, "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "kernel": "trailing-update GEMM (bf16 MFMA, fp32 acc)",
+                     "achieved": round(achieved / 1e12, 2), "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
+                     "launches": launches, "avg_launch_ms": round(gms / max(launches, 1), 4),
+                     "avg_launch_gflop": round(gflop / max(launches, 1) / 1e9, 3)},
+        "backward_error": sb["backward_error"],
+        "forward_error_vs_fp64_truth": sb["forward_error"],
+        "gmres_iterations": sb["gmres_iterations"],
+        "fp32_ms_per_step": round(res["SS"]["ms"], 3),
+        "fp32_backward_error": res["SS"]["backward_error"],
+        "fp32_GFLOP_per_s": round(flops / (res["SS"]["ms"] * 1e-3) / 1e9, 1),
+        "speedup_vs_fp32": round(res["SS"]["ms"] / sb["ms"], 3),
+        "fp16_ir": {"ms": round(res["SH"]["ms"], 3), "GFLOP_per_s": round(flops / (res["SH"]["ms"] * 1e-3) / 1e9, 1),
+                    "backward_error": res["SH"]["backward_error"], "gmres_iterations": res["SH"]["gmres_iterations"]},
+    }
+    del at
+    torch.cuda.empty_cache()
+    if not args.skip_batched:
+        log("batched inverse 65536 x 32x32")
+        out["batched_inverse"] = batched_inverse(A, torch, h)
+    if not args.skip_cpu:
+        log(f"host LAPACK baseline n={args.cpu_n}")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    h.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

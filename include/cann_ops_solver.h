@@ -130,6 +130,21 @@ extern "C"
     aclsolverStatus_t aclsolverSgesvVbatched(aclsolverHandle_t handle, int batchSize, const int* nArray, int nrhs,
                                              float* const dAarray[], float* const dBarray[], int* dIpiv, int* dInfo);
 
+    /* ---- EXTENSION: contiguous strided batched direct inverse, n <= 64, with optional
+     * per-matrix pivots (n*batch ints). No metadata validation pass and no host sync; used by
+     * measurement harnesses and by callers that own the contract. */
+    aclsolverStatus_t aclsolverSmatinvStridedBatched(aclsolverHandle_t handle, int n, const float* A, int lda,
+                                                     long long strideA, float* Ainv, int lda_inv, long long strideInv,
+                                                     int* ipiv, int* info, int batchSize);
+
+    /* ---- EXTENSION (measurement): per-launch timing of instrumented kernels. Between
+     * Begin and End, launches of kernel class k are bracketed by HIP events on their own
+     * stream; End reports the launch count, summed algorithmic work (flop or byte) and the
+     * summed duration (ms). k = 0: trailing-update GEMM of getrf; 1: batched inverse;
+     * 2: uneven batched getrf+getrs. */
+    aclsolverStatus_t aclsolverProfileBegin(void);
+    aclsolverStatus_t aclsolverProfileEnd(int kernel, int* launches, double* work, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

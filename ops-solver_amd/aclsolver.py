@@ -79,6 +79,8 @@ def lib():
         "aclsolverSgesvVbatched_execute": ([P, P, P, P, P, P], S),
         "aclsolverSgesvVbatched_destroyPlan": ([P], S),
         "aclsolverSgesvVbatched": ([P, I, P, I, P, P, P, P], S),
+        "aclsolverProfileBegin": ([], S),
+        "aclsolverProfileEnd": ([I, P, P, P], S),
     }
     for name, (args, res) in sig.items():
         if hasattr(L, name):

@@ -1,0 +1,117 @@
+// EXTENSION entry points: the mixed-precision gesv family with GMRES refinement.
+// Argument convention follows the reference's aclsolverSSgels (include/cann_ops_solver.h:815-862).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.h"
+
+namespace ops
+{
+size_t GesvIrWorkspaceBytes(int n, int restart);
+hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
+                  void* work, Precision prec, double tol, int maxOuter, int restart, int* info, int* niterOut,
+                  hipStream_t stream);
+}  // namespace ops
+
+using namespace ops;
+
+namespace
+{
+constexpr int kGesvMax = 16384;
+
+aclsolverStatus_t GesvBufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* bytes)
+{
+    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (n < 0 || n > kGesvMax || nrhs < 0 || bytes == nullptr) return ACLSOLVER_STATUS_INVALID_VALUE;
+    *bytes = n == 0 ? 0 : GesvIrWorkspaceBytes(n, H(handle)->irRestart);
+    return ACLSOLVER_STATUS_SUCCESS;
+}
+
+// Argument indices (handle excluded): n=1 nrhs=2 dA=3 ldda=4 dipiv=5 dB=6 lddb=7 dX=8
+// lddx=9 dWorkspace=10 lwork_bytes=11 niter=12 dinfo=13.
+aclsolverStatus_t Gesv(aclsolverHandle_t handle, Precision prec, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                       float* dB, int lddb, float* dX, int lddx, void* dWork, size_t lwork, int* niter, int* dinfo)
+{
+    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (dinfo == nullptr || !IsDevice(dinfo)) return ACLSOLVER_STATUS_INVALID_VALUE;
+    Handle* h = H(handle);
+    int arg = 0;
+    const size_t need = n > 0 ? GesvIrWorkspaceBytes(n, h->irRestart) : 0;
+    if (n < 0 || n > kGesvMax)
+        arg = 1;
+    else if (nrhs < 0)
+        arg = 2;
+    else if (n > 0 && !IsDevice(dA))
+        arg = 3;
+    else if (ldda < std::max(1, n))
+        arg = 4;
+    else if (dipiv != nullptr && !IsDevice(dipiv))
+        arg = 5;
+    else if (n > 0 && nrhs > 0 && !IsDevice(dB))
+        arg = 6;
+    else if (lddb < std::max(1, n))
+        arg = 7;
+    else if (n > 0 && nrhs > 0 && (!IsDevice(dX) || dX == dB || dX == dA))
+        arg = 8;
+    else if (lddx < std::max(1, n))
+        arg = 9;
+    else if (n > 0 && !IsDevice(dWork))
+        arg = 10;
+    else if (lwork < need)
+        arg = 11;
+    else if (niter == nullptr)
+        arg = 12;
+    if (arg) return Invalid(handle, dinfo, 1, arg);
+    hipStream_t s = h->stream;
+    *niter = 0;
+    FillInfo(dinfo, 1, 0, s);
+    if (n == 0 || nrhs == 0) return Launched();
+    // dipiv is optional; keep pivots inside the workspace tail when absent.
+    int* piv = dipiv;
+    if (piv == nullptr)
+    {
+        void* extra = HandleScratch(handle, static_cast<size_t>(n) * sizeof(int));
+        if (extra == nullptr) return ACLSOLVER_STATUS_ALLOC_FAILED;
+        piv = static_cast<int*>(extra);
+    }
+    if (GesvIR(n, nrhs, dA, ldda, piv, dB, lddb, dX, lddx, dWork, prec, h->irTolerance, h->irMaxOuter, h->irRestart,
+               dinfo, niter, s) != hipSuccess)
+        return ACLSOLVER_STATUS_EXECUTION_FAILED;
+    return Launched();
+}
+}  // namespace
+
+OPS_API aclsolverStatus_t aclsolverSHgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes)
+{
+    return GesvBufferSize(handle, n, nrhs, lwork_bytes);
+}
+OPS_API aclsolverStatus_t aclsolverSBgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes)
+{
+    return GesvBufferSize(handle, n, nrhs, lwork_bytes);
+}
+OPS_API aclsolverStatus_t aclsolverSSgesv_bufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* lwork_bytes)
+{
+    return GesvBufferSize(handle, n, nrhs, lwork_bytes);
+}
+OPS_API aclsolverStatus_t aclsolverSHgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                          float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
+                                          size_t lwork_bytes, int* niter, int* dinfo)
+{
+    return Gesv(handle, Precision::FP16, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
+                dinfo);
+}
+OPS_API aclsolverStatus_t aclsolverSBgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                          float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
+                                          size_t lwork_bytes, int* niter, int* dinfo)
+{
+    return Gesv(handle, Precision::BF16, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
+                dinfo);
+}
+OPS_API aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
+                                          float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
+                                          size_t lwork_bytes, int* niter, int* dinfo)
+{
+    return Gesv(handle, Precision::FP32, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
+                dinfo);
+}

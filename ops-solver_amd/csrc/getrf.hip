@@ -460,9 +460,12 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
             RET_IF(TrsmBlock(nb, right, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, false,
                              a + static_cast<size_t>(panelEnd) * lda + j0, lda, stream));
             if (m - panelEnd > 0)
+            {
+                ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * (m - panelEnd) * right * nb, stream);
                 RET_IF(GemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
                                a + static_cast<size_t>(panelEnd) * lda + j0, lda,
                                a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, stream));
+            }
         }
     }
     return hipGetLastError();

@@ -56,7 +56,31 @@ bool AllFinite(aclsolverHandle_t h, const float* a, int rows, int cols, int lda,
 bool AllFiniteArrays(aclsolverHandle_t h, const float* const* arrays, int rows, int cols, int lda, int batch,
                      bool* ok);
 
-// ---- real LU engine (lu_engine.hip) ----
+// ---- optional launch timing (profile.hip) ----
+enum class ProfileKernel
+{
+    TrailingGemm = 0,
+    BatchedInverse = 1,
+    Vbatched = 2
+};
+bool ProfilingOn();
+// Brackets the launches issued during its lifetime with HIP events on `stream`.
+class ProfileScope
+{
+   public:
+    ProfileScope(ProfileKernel k, double work, hipStream_t stream);
+    ~ProfileScope();
+    ProfileScope(const ProfileScope&) = delete;
+    ProfileScope& operator=(const ProfileScope&) = delete;
+
+   private:
+    ProfileKernel kernel_;
+    double work_;
+    hipStream_t stream_;
+    hipEvent_t start_ = nullptr;
+};
+
+// ---- real LU engine (getrf.hip) ----
 enum class Precision
 {
     FP32 = 0,

@@ -440,7 +440,9 @@ static cf csubmul_f(cf x, cf a, cf b)
     return r;
 }
 
-ORC_EXPORT int orc_cgetrf(int m, int n, float *af, int lda, int *ipiv)
+/* margin (nullable): per step, relative gap of |re|+|im| between the winner and the best
+ * other candidate (the tie margin a parity test filters on). */
+ORC_EXPORT int orc_cgetrf(int m, int n, float *af, int lda, int *ipiv, float *margin)
 {
     cf *a = (cf *)af;
     int info = 0;
@@ -460,6 +462,16 @@ ORC_EXPORT int orc_cgetrf(int m, int n, float *af, int lda, int *ipiv)
                     best = s;
                     p = i;
                 }
+            }
+            if (margin)
+            {
+                double w = fabs((double)col[p].re) + fabs((double)col[p].im), second = 0.0;
+                for (int i = k; i < m; ++i)
+                {
+                    double v = fabs((double)col[i].re) + fabs((double)col[i].im);
+                    if (i != p && v > second) second = v;
+                }
+                margin[k] = w > 0.0 ? (float)((w - second) / w) : 0.0f;
             }
             ipiv[k] = p + 1;
             if (p != k)
@@ -616,7 +628,7 @@ ORC_EXPORT int orc_cmatinv(int n, const float *af, int lda, float *cfp, int ldc,
     cf *c = (cf *)cfp;
     int *piv = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     for (int j = 0; j < n; ++j) memcpy(c + (size_t)j * ldc, a + (size_t)j * lda, sizeof(cf) * (size_t)n);
-    int info = orc_cgetrf(n, n, cfp, ldc, piv);
+    int info = orc_cgetrf(n, n, cfp, ldc, piv, NULL);
     if (info == 0) orc_cgetri(n, cfp, ldc, piv);
     if (ipiv_out) memcpy(ipiv_out, piv, sizeof(int) * (size_t)n);
     free(piv);

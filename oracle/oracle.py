@@ -39,7 +39,7 @@ def lib():
         _lib.orc_dgetrf.argtypes = [i, i, p, i, p]
         _lib.orc_dgetrf.restype = i
         _lib.orc_dgetrs.argtypes = [i, i, i, p, i, p, p, i]
-        _lib.orc_cgetrf.argtypes = [i, i, p, i, p]
+        _lib.orc_cgetrf.argtypes = [i, i, p, i, p, p]
         _lib.orc_cgetrf.restype = i
         _lib.orc_cgetrs.argtypes = [i, i, i, p, i, p, p, i]
         _lib.orc_cgetri.argtypes = [i, p, i, p]
@@ -125,12 +125,15 @@ def smatinv(a):
     return c, piv[:n], info
 
 
-def cgetrf(a, pivot=True):
+def cgetrf(a, pivot=True, margins=False):
     lu = _fortran(a, np.complex64)
     m, n = lu.shape
     k = min(m, n)
     ipiv = np.zeros(max(k, 1), np.int32) if pivot else None
-    info = lib().orc_cgetrf(m, n, _ptr(lu), max(m, 1), _ptr(ipiv))
+    margin = np.ones(max(k, 1), np.float32) if margins else None
+    info = lib().orc_cgetrf(m, n, _ptr(lu), max(m, 1), _ptr(ipiv), _ptr(margin))
+    if margins:
+        return lu, (ipiv[:k] if pivot else None), info, margin[:k]
     return lu, (ipiv[:k] if pivot else None), info
 
 

@@ -74,9 +74,11 @@ def test_sgetrf_parity(torch, handle, m, n, seed):
 
 
 def test_sgetrf_config1_solve_vs_fp64(torch, handle):
-    """Config: n=512, A = N(0,1)/sqrt(n), one RHS b ~ N(0,1), partial pivoting."""
+    """Config: n=512, A = N(0,1)/sqrt(n), one RHS b ~ N(0,1), partial pivoting. Seed 513 is
+    tie-free (every step's top-2 pivot candidates differ by > 1e-4 relative), so ipiv must
+    match the oracle over all 512 steps."""
     n = 512
-    rng = np.random.default_rng(512)
+    rng = np.random.default_rng(513)
     a = (rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32)
     b = rng.standard_normal((n, 1)).astype(np.float32)
     st, lu, piv, info, d = _sgetrf(torch, handle, a)
