@@ -33,7 +33,7 @@ class LibraryMissing(RuntimeError):
 def declared_symbols():
     """Entry points declared in include/cann_ops_solver.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(aclsolver\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"(?:aclsolverStatus_t|aclError)\s+(aclsolver\w+)\s*\(", text)))
 
 
 def lib():
