@@ -112,6 +112,10 @@ extern "C"
     /* EXTENSION: refinement controls for the gesv family (tolerance on the backward error,
      * maximum outer refinement steps, GMRES restart length). */
     aclsolverStatus_t aclsolverSetRefinement(aclsolverHandle_t handle, double tolerance, int maxOuter, int restart);
+    /* EXTENSION: 1 (default) = SBgesv's trailing updates use split bf16 operands
+     * (x = hi + lo, products hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation);
+     * 0 = single bf16 operands. */
+    aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int split);
 
     /* ---- EXTENSION: uneven batched LU solve (getrf + getrs) in one persistent launch ----
      * nArray (HOST, batchSize entries) holds each order n_i; dAarray / dBarray are Device

@@ -105,8 +105,17 @@ OPS_API aclsolverStatus_t aclsolverSBgesv(aclsolverHandle_t handle, int n, int n
                                           float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
                                           size_t lwork_bytes, int* niter, int* dinfo)
 {
-    return Gesv(handle, Precision::BF16, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
-                dinfo);
+    const Precision p = (handle != nullptr && H(handle)->bf16Split) ? Precision::BF16X3 : Precision::BF16;
+    return Gesv(handle, p, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter, dinfo);
+}
+
+// EXTENSION: 1 (default) = SBgesv's trailing updates use the split-bf16 form
+// (hi*hi + hi*lo + lo*hi on bf16 MFMA, ~16-bit products); 0 = plain bf16 inputs.
+OPS_API aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int split)
+{
+    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    H(handle)->bf16Split = split != 0;
+    return ACLSOLVER_STATUS_SUCCESS;
 }
 OPS_API aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
                                           float* dB, int lddb, float* dX, int lddx, void* dWorkspace,

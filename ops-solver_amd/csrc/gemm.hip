@@ -296,6 +296,117 @@ __global__ __launch_bounds__(256) void gemm_sub_half_kernel(int M, int N, int K,
         }
 }
 
+// Split BF16 form: every FP32 operand x is held as hi = bf16(x), lo = bf16(x - hi) and the
+// product is accumulated as hi*hi + hi*lo + lo*hi on bf16 MFMA with FP32 accumulation
+// (about 16 significant bits per product instead of 8). Three MFMAs per block pair; the
+// trailing update stays memory-bound, so the extra matrix-core work is mostly hidden.
+template <int TRANSA>
+__global__ __launch_bounds__(256) void gemm_sub_bf16x3_kernel(int M, int N, int K, const float* __restrict__ A,
+                                                              int lda, const float* __restrict__ B, int ldb,
+                                                              float* __restrict__ C, int ldc, int tilesM, int tilesN)
+{
+    typedef __bf16 hx8 __attribute__((ext_vector_type(8)));
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[TM][32 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TM][32 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bh[TN][32 + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bl[TN][32 + 8];
+    int tm, tn;
+    TileOf(tilesM, tilesN, &tm, &tn);
+    const int row0 = tm * TM, col0 = tn * TN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+    auto split = [](float v, __bf16& h, __bf16& l)
+    {
+        h = static_cast<__bf16>(v);
+        l = static_cast<__bf16>(v - static_cast<float>(h));
+    };
+    for (int k0 = 0; k0 < K; k0 += 32)
+    {
+        if (!TRANSA)
+        {
+            const int r = tid & 127, kq = (tid >> 7) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                const int k = kq + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(k0 + k) * lda + row0 + r];
+                split(v, Ah[r][k], Al[r][k]);
+            }
+        }
+        else
+        {
+            const int r = tid >> 1, kh = (tid & 1) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                const int k = kh + kk;
+                float v = 0.0f;
+                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(row0 + r) * lda + k0 + k];
+                split(v, Ah[r][k], Al[r][k]);
+            }
+        }
+        {
+            const int j = tid >> 1, kh = (tid & 1) * 16;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+            {
+                const int k = kh + kk;
+                float v = 0.0f;
+                if (col0 + j < N && k0 + k < K) v = B[static_cast<size_t>(col0 + j) * ldb + k0 + k];
+                split(v, Bh[j][k], Bl[j][k]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 16)
+        {
+            const int kq = ks + (lane >> 5) * 8;
+            const int ra = wm * 64 + (lane & 31), rb = wn * 64 + (lane & 31);
+            const hx8 a0h = *reinterpret_cast<const hx8*>(&Ah[ra][kq]);
+            const hx8 a1h = *reinterpret_cast<const hx8*>(&Ah[ra + 32][kq]);
+            const hx8 a0l = *reinterpret_cast<const hx8*>(&Al[ra][kq]);
+            const hx8 a1l = *reinterpret_cast<const hx8*>(&Al[ra + 32][kq]);
+            const hx8 b0h = *reinterpret_cast<const hx8*>(&Bh[rb][kq]);
+            const hx8 b1h = *reinterpret_cast<const hx8*>(&Bh[rb + 32][kq]);
+            const hx8 b0l = *reinterpret_cast<const hx8*>(&Bl[rb][kq]);
+            const hx8 b1l = *reinterpret_cast<const hx8*>(&Bl[rb + 32][kq]);
+#define BF3(ACC, AH, AL, BH, BL)                                                \
+    ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(AL, BH, ACC, 0, 0, 0);        \
+    ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(AH, BL, ACC, 0, 0, 0);        \
+    ACC = __builtin_amdgcn_mfma_f32_32x32x16_bf16(AH, BH, ACC, 0, 0, 0)
+            BF3(acc[0][0], a0h, a0l, b0h, b0l);
+            BF3(acc[0][1], a0h, a0l, b1h, b1l);
+            BF3(acc[1][0], a1h, a1l, b0h, b0l);
+            BF3(acc[1][1], a1h, a1l, b1h, b1l);
+#undef BF3
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+        {
+            const int j = col0 + wn * 64 + bj * 32 + (lane & 31);
+            if (j >= N) continue;
+            float* cj = C + static_cast<size_t>(j) * ldc;
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+            {
+                const int i = row0 + wm * 64 + bi * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                if (i < M) cj[i] -= acc[bi][bj][v];
+            }
+        }
+}
+
 // B <- op(T)^{-1} B for a K x K triangle. One workgroup owns CT columns of B in LDS; the
 // row that step s completes is divided by its diagonal by the thread that applied the
 // last update to it, so each step needs a single barrier.
@@ -398,6 +509,15 @@ hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, con
         else
             hipLaunchKernelGGL((gemm_sub_half_kernel<_Float16, 0, true>), dim3(blocks), dim3(256), 0, stream, M, N, K,
                                A, lda, B, ldb, C, ldc, tilesM, tilesN);
+    }
+    else if (prec == Precision::BF16X3)
+    {
+        if (transA)
+            hipLaunchKernelGGL(gemm_sub_bf16x3_kernel<1>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb,
+                               C, ldc, tilesM, tilesN);
+        else
+            hipLaunchKernelGGL(gemm_sub_bf16x3_kernel<0>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb,
+                               C, ldc, tilesM, tilesN);
     }
     else
     {

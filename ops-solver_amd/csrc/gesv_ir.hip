@@ -28,6 +28,12 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
                          int* permOut, hipStream_t stream);
 hipError_t RealLuSolveInPlace(int trans, int n, int nrhs, const float* lu, int lda, float* x, int ldx,
                               hipStream_t stream);
+hipError_t RbtMakeDiagonals(int n, unsigned long long seed, float* du, float* dv, hipStream_t s);
+hipError_t RbtTransform(int n, const float* A, int lda, float* dst, int ldd, const float* du, const float* dv,
+                        hipStream_t s);
+hipError_t RbtApplyVecLeft(int n, const double* x, double* y, const float* du, hipStream_t s);
+hipError_t RbtApplyVecRight(int n, const double* x, double* y, const float* dv, hipStream_t s);
+hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing, hipStream_t s);
 
 namespace
 {
@@ -134,7 +140,7 @@ __global__ void prep_rhs_kernel(int n, const double* __restrict__ v, const int* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double s = scal[slot] > 0.0 ? scal[slot] : 1.0;
-    y[i] = static_cast<float>(v[perm[i]] / s);
+    y[i] = static_cast<float>(v[perm ? perm[i] : i] / s);
 }
 
 __global__ void finish_solve_kernel(int n, const float* __restrict__ y, const double* __restrict__ scal, int slot,
@@ -155,6 +161,12 @@ __global__ void correct_kernel(int n, int k, const double* __restrict__ V, int l
     double s = x[i];
     for (int q = 0; q < k; ++q) s = fma(V[static_cast<size_t>(q) * ldv + i], y[q], s);
     x[i] = s;
+}
+
+__global__ void identity_ipiv_kernel(int n, int* __restrict__ ipiv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ipiv[i] = i + 1;
 }
 
 __global__ void f32_to_f64_kernel(int n, const float* __restrict__ a, int64_t stride, double* __restrict__ b)
@@ -185,7 +197,8 @@ struct IrWork
     int* perm;
     void* getrfWork;
     float* y32;
-    double *x, *b, *r, *w, *V, *partial, *scal, *small;
+    double *x, *b, *r, *w, *V, *partial, *scal, *small, *t64;
+    float *du, *dv;  // random butterfly diagonals (2n each)
     size_t bytes;
 };
 
@@ -215,6 +228,9 @@ IrWork IrLayout(void* base, int n, int restart)
     w.partial = reinterpret_cast<double*>(take(static_cast<size_t>(chunks) * n * sizeof(double)));
     w.scal = reinterpret_cast<double*>(take(64 * sizeof(double)));
     w.small = reinterpret_cast<double*>(take(static_cast<size_t>(restart + 2) * 4 * sizeof(double)));
+    w.t64 = reinterpret_cast<double*>(take(static_cast<size_t>(n) * sizeof(double)));
+    w.du = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
+    w.dv = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
     w.bytes = off;
     return w;
 }
@@ -244,17 +260,47 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
 {
     IrWork w = IrLayout(work, n, restart);
     const int chunks = (n + CB - 1) / CB;
-    hipLaunchKernelGGL(copy_cm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, stream, n, n, A, lda, w.lu, n);
-    hipError_t e = RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream);
-    if (e != hipSuccess) return e;
+    // The mixed precisions factor U^T A V without pivoting (random butterflies, n % 4 == 0);
+    // FP32, other orders and the fallback after a refinement failure use partial pivoting.
+    bool useRbt = prec != Precision::FP32 && n % 4 == 0 && n >= 8;
+    hipError_t e = hipSuccess;
+    auto factor = [&]() -> hipError_t
+    {
+        if (useRbt)
+        {
+            hipError_t err = RbtMakeDiagonals(n, 0x5EED0000ull + static_cast<unsigned long long>(n), w.du, w.dv, stream);
+            if (err == hipSuccess) err = RbtTransform(n, A, lda, w.lu, n, w.du, w.dv, stream);
+            if (err == hipSuccess) err = GetrfNoPivot(n, w.lu, n, info, prec, stream);
+            hipLaunchKernelGGL(identity_ipiv_kernel, G1(n), dim3(256), 0, stream, n, ipiv);
+            return err == hipSuccess ? hipGetLastError() : err;
+        }
+        hipLaunchKernelGGL(copy_cm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, stream, n, n, A, lda, w.lu, n);
+        return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream);
+    };
+    if ((e = factor()) != hipSuccess) return e;
     const dim3 gemvGrid((n + RB - 1) / RB, chunks);
     auto solveM = [&](const double* v, double* out, int slot) -> hipError_t
     {
-        // out = M^-1 v with M = P^T L U, applied in FP32 on the max-scaled vector.
-        hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, v, w.scal, slot);
-        hipLaunchKernelGGL(prep_rhs_kernel, G1(n), dim3(256), 0, stream, n, v, w.perm, w.scal, slot, w.y32);
+        // out = M^-1 v, M = P^T L U (pivoted) or U^-T L U V^-1 (butterflies), with the LU
+        // solves in FP32 on the max-scaled vector.
+        const double* src = v;
+        if (useRbt)
+        {
+            hipError_t err = RbtApplyVecLeft(n, v, w.t64, w.du, stream);
+            if (err != hipSuccess) return err;
+            src = w.t64;
+        }
+        hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, src, w.scal, slot);
+        hipLaunchKernelGGL(prep_rhs_kernel, G1(n), dim3(256), 0, stream, n, src, useRbt ? nullptr : w.perm, w.scal,
+                           slot, w.y32);
         hipError_t err = RealLuSolveInPlace(0, n, 1, w.lu, n, w.y32, n, stream);
-        hipLaunchKernelGGL(finish_solve_kernel, G1(n), dim3(256), 0, stream, n, w.y32, w.scal, slot, out);
+        if (useRbt)
+        {
+            hipLaunchKernelGGL(finish_solve_kernel, G1(n), dim3(256), 0, stream, n, w.y32, w.scal, slot, w.t64);
+            if (err == hipSuccess) err = RbtApplyVecRight(n, w.t64, out, w.dv, stream);
+        }
+        else
+            hipLaunchKernelGGL(finish_solve_kernel, G1(n), dim3(256), 0, stream, n, w.y32, w.scal, slot, out);
         return err;
     };
     auto residual = [&](const double* x, double* r)
@@ -281,9 +327,24 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
         hipLaunchKernelGGL(f32_to_f64_kernel, G1(n), dim3(256), 0, stream, n, B + static_cast<size_t>(c) * ldb, 1,
                            w.b);
         hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.b, w.scal, S_BNORM);
+    restart_rhs:
         if ((e = solveM(w.b, w.x, S_VSCALE)) != hipSuccess) return e;
         bool done = prec == Precision::FP32;
         int outer = 0;
+        if (useRbt)
+        {
+            // A zero pivot without pivoting means the butterflies did not help: pivot instead.
+            int hinfo = 0;
+            if ((e = hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(stream)) != hipSuccess)
+                return e;
+            if (hinfo != 0)
+            {
+                useRbt = false;
+                if ((e = factor()) != hipSuccess) return e;
+                goto restart_rhs;
+            }
+        }
         while (!done)
         {
             residual(w.x, w.r);
@@ -297,6 +358,12 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
             if (!(berr > tol)) break;
             if (outer++ >= maxOuter)
             {
+                if (useRbt)  // refinement stalled on the unpivoted factors: refactor with pivoting
+                {
+                    useRbt = false;
+                    if ((e = factor()) != hipSuccess) return e;
+                    goto restart_rhs;
+                }
                 converged = false;
                 break;
             }

@@ -23,6 +23,7 @@ struct Handle
     double irTolerance = 1e-6;
     int irMaxOuter = 30;
     int irRestart = 30;
+    int bf16Split = 1;        // SBgesv: split-bf16 (3-product) trailing updates
     void* scratch = nullptr;  // handle-owned Device scratch (grown on demand)
     int* dflag = nullptr;     // Device flag word for validation scans
     int* hflag = nullptr;     // pinned host mirror of dflag
@@ -85,7 +86,8 @@ enum class Precision
 {
     FP32 = 0,
     FP16 = 1,
-    BF16 = 2
+    BF16 = 2,
+    BF16X3 = 3  // split bf16 (hi*hi + hi*lo + lo*hi) on bf16 MFMA
 };
 // Workspace bytes needed by RealGetrf for an m x n matrix.
 size_t RealGetrfWorkspaceBytes(int m, int n);

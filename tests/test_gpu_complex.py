@@ -157,8 +157,8 @@ def test_cmatinv_batched(torch, handle, n, batch):
     dA = [A.to_device_cm(m) for m in mats]
     dC = [torch.zeros((n, n), dtype=torch.complex64, device="cuda") for _ in range(batch)]
     info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
-    st = A.lib().aclsolverCmatinvBatched(handle.h, n, A.ptr(_ptr_array(torch, dA)), n, A.ptr(_ptr_array(torch, dC)),
-                                         n, A.ptr(info), batch)
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)  # kept alive across the call
+    st = A.lib().aclsolverCmatinvBatched(handle.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
     torch.cuda.synchronize()
     assert st == A.SUCCESS and (info.cpu().numpy() == 0).all()
     for m, c in zip(mats, dC):
@@ -175,8 +175,8 @@ def test_cgetri_batched_and_single(torch, handle, n, batch):
     piv = torch.from_numpy(np.concatenate([f[1] for f in facts]).astype(np.int32)).cuda()
     dC = [torch.zeros((n, n), dtype=torch.complex64, device="cuda") for _ in range(batch)]
     info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
-    st = A.lib().aclsolverCgetriBatched(handle.h, n, A.ptr(_ptr_array(torch, dA)), n, A.ptr(piv),
-                                        A.ptr(_ptr_array(torch, dC)), n, A.ptr(info), batch)
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)
+    st = A.lib().aclsolverCgetriBatched(handle.h, n, A.ptr(pa), n, A.ptr(piv), A.ptr(pc), n, A.ptr(info), batch)
     torch.cuda.synchronize()
     assert st == A.SUCCESS and (info.cpu().numpy() == 0).all()
     for m, c in zip(mats, dC):

@@ -249,8 +249,9 @@ def test_smatinv_batched(torch, handle, n, batch):
     dA = [A.to_device_cm(m) for m in mats]
     dC = [torch.zeros((n, n), device="cuda") for _ in range(batch)]
     info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
-    st = A.lib().aclsolverSmatinvBatched(handle.h, n, A.ptr(_ptr_array(torch, dA)), n,
-                                         A.ptr(_ptr_array(torch, dC)), n, A.ptr(info), batch)
+    # keep the pointer arrays alive: a freed temporary would be recycled by the allocator
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)
+    st = A.lib().aclsolverSmatinvBatched(handle.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
     torch.cuda.synchronize()
     assert st == A.SUCCESS
     assert (info.cpu().numpy() == 0).all()
@@ -296,8 +297,8 @@ def test_sgetri_batched(torch, handle, n, batch):
     piv = torch.from_numpy(np.concatenate([f[1] for f in facts]).astype(np.int32)).cuda()
     dC = [torch.zeros((n, n + 1), device="cuda") for _ in range(batch)]
     info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
-    st = A.lib().aclsolverSgetriBatched(handle.h, n, A.ptr(_ptr_array(torch, dA)), n, A.ptr(piv),
-                                        A.ptr(_ptr_array(torch, dC)), n + 1, A.ptr(info), batch)
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)
+    st = A.lib().aclsolverSgetriBatched(handle.h, n, A.ptr(pa), n, A.ptr(piv), A.ptr(pc), n + 1, A.ptr(info), batch)
     torch.cuda.synchronize()
     assert st == A.SUCCESS and (info.cpu().numpy() == 0).all()
     for m, c in zip(mats, dC):
