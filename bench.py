@@ -80,9 +80,11 @@ def gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo):
 
 def time_mode(A, torch, h, name, steps, warmup, n, at, b, x, work, lwork, dipiv, dinfo, profile=False):
     iters = 0
-    for _ in range(warmup):
-        gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo)
-    torch.cuda.synchronize()
+    for wi in range(warmup):
+        t0 = time.perf_counter()
+        it = gesv(A, torch, h, name, n, at, b, x, work, lwork, dipiv, dinfo)
+        torch.cuda.synchronize()
+        log(f"{name} warmup {wi + 1}/{warmup}: {(time.perf_counter() - t0) * 1e3:.1f} ms, {it} GMRES iterations")
     if profile:
         A.lib().aclsolverProfileBegin()
     t0 = time.perf_counter()
@@ -238,7 +240,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=8192)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
-    ap.add_argument("--modes", default="SS,SH,SB", help="gesv variants to time (SB is the headline)")
+    ap.add_argument("--modes", default="SS,SB", help="gesv variants to time: SS, SH, SB (SB, the headline, always runs)")
     ap.add_argument("--restart", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:

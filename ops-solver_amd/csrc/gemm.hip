@@ -10,6 +10,7 @@
 // power of two chosen from the tile's largest magnitude (exact, undone after the sum).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "lu_kernels.h"
@@ -407,81 +408,68 @@ __global__ __launch_bounds__(256) void gemm_sub_bf16x3_kernel(int M, int N, int 
         }
 }
 
-// B <- op(T)^{-1} B for a K x K triangle. One workgroup owns CT columns of B in LDS; the
-// row that step s completes is divided by its diagonal by the thread that applied the
-// last update to it, so each step needs a single barrier.
-template <int CT, bool LOWER, bool UNIT, bool TRANS>
-__global__ __launch_bounds__(256) void trsm_block_kernel(int K, int N, const float* __restrict__ T, int ldt,
-                                                         float* __restrict__ B, int ldb)
+// B <- op(T)^{-1} B on one diagonal block of at most 64 rows. Every thread owns one column
+// of B in registers and runs the substitution on it alone (no barriers, no exchange);
+// the block of op(T) sits in LDS, read by all lanes at the same address (broadcast). The
+// update order per element is the LAPACK strsm order: ascending (forward) or descending
+// (backward) pivot index, one fused multiply-add per term, one division per pivot.
+template <bool FWD, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const float* __restrict__ T, int ldt,
+                                                          int transT, float* __restrict__ B, int ldb)
 {
-    extern __shared__ float sB[];  // K x (CT + 1)
-    constexpr int G = 256 / CT;    // row groups
-    constexpr bool FWD = LOWER != TRANS;
-    const int tid = threadIdx.x, c = tid % CT, g = tid / CT;
-    const int colBase = blockIdx.x * CT;
-    const bool active = colBase + c < N;
-    float* bcol = B + static_cast<size_t>(colBase + c) * ldb;
-    auto opT = [&](int i, int s) -> float
-    { return TRANS ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i]; };
-    for (int i = g; i < K; i += G) sB[i * (CT + 1) + c] = active ? bcol[i] : 0.0f;
-    __syncthreads();
-    const int first = FWD ? 0 : K - 1;
-    if (!UNIT && g == 0 && active) sB[first * (CT + 1) + c] /= T[static_cast<size_t>(first) * ldt + first];
-    __syncthreads();
-    if (FWD)
+    __shared__ __attribute__((aligned(16))) float Ts[64][64];  // Ts[s][i] = op(T)(i, s)
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        for (int s = 0; s < K - 1; ++s)
+        const int s = e >> 6, i = e & 63;
+        float v = 0.0f;
+        if (i < kb && s < kb) v = transT ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i];
+        Ts[s][i] = v;
+    }
+    __syncthreads();
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= N) return;
+    float* b = B + static_cast<size_t>(col) * ldb;
+    float x[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[i] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 64; ++t)
+    {
+        const int s = FWD ? t : 63 - t;
+        if (s < kb)
         {
-            const float x = sB[s * (CT + 1) + c];
-            for (int i = s + 1 + g; i < K; i += G)
+            if (!UNIT) x[s] = x[s] / Ts[s][s];
+            if (FWD)
             {
-                float v = fmaf(-opT(i, s), x, sB[i * (CT + 1) + c]);
-                if (!UNIT && i == s + 1) v = v / T[static_cast<size_t>(i) * ldt + i];
-                sB[i * (CT + 1) + c] = v;
+#pragma unroll
+                for (int i = s + 1; i < 64; ++i) x[i] = fmaf(-Ts[s][i], x[s], x[i]);
             }
-            __syncthreads();
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < s; ++i) x[i] = fmaf(-Ts[s][i], x[s], x[i]);
+            }
         }
     }
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        if (i < kb) b[i] = x[i];
+}
+
+hipError_t TrsmDiag(int kb, int N, const float* T, int ldt, bool fwd, bool unit, bool transT, float* B, int ldb,
+                    hipStream_t stream)
+{
+    const dim3 grid((N + 255) / 256);
+    const int tr = transT ? 1 : 0;
+    if (fwd && unit)
+        hipLaunchKernelGGL((trsm_diag64_kernel<true, true>), grid, dim3(256), 0, stream, kb, N, T, ldt, tr, B, ldb);
+    else if (fwd)
+        hipLaunchKernelGGL((trsm_diag64_kernel<true, false>), grid, dim3(256), 0, stream, kb, N, T, ldt, tr, B, ldb);
+    else if (unit)
+        hipLaunchKernelGGL((trsm_diag64_kernel<false, true>), grid, dim3(256), 0, stream, kb, N, T, ldt, tr, B, ldb);
     else
-    {
-        for (int s = K - 1; s > 0; --s)
-        {
-            const float x = sB[s * (CT + 1) + c];
-            for (int i = s - 1 - g; i >= 0; i -= G)
-            {
-                float v = fmaf(-opT(i, s), x, sB[i * (CT + 1) + c]);
-                if (!UNIT && i == s - 1) v = v / T[static_cast<size_t>(i) * ldt + i];
-                sB[i * (CT + 1) + c] = v;
-            }
-            __syncthreads();
-        }
-    }
-    if (active)
-        for (int i = g; i < K; i += G) bcol[i] = sB[i * (CT + 1) + c];
-}
-
-template <int CT, bool LOWER, bool UNIT, bool TRANS>
-hipError_t LaunchTrsm(int K, int N, const float* T, int ldt, float* B, int ldb, hipStream_t stream)
-{
-    size_t shmem = static_cast<size_t>(K) * (CT + 1) * sizeof(float);
-    int blocks = (N + CT - 1) / CT;
-    hipLaunchKernelGGL((trsm_block_kernel<CT, LOWER, UNIT, TRANS>), dim3(blocks), dim3(256), shmem, stream, K, N, T,
-                       ldt, B, ldb);
+        hipLaunchKernelGGL((trsm_diag64_kernel<false, false>), grid, dim3(256), 0, stream, kb, N, T, ldt, tr, B, ldb);
     return hipGetLastError();
-}
-
-template <int CT>
-hipError_t TrsmDispatch(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
-                        hipStream_t stream)
-{
-    if (lower && unit && !transT) return LaunchTrsm<CT, true, true, false>(K, N, T, ldt, B, ldb, stream);
-    if (lower && unit && transT) return LaunchTrsm<CT, true, true, true>(K, N, T, ldt, B, ldb, stream);
-    if (lower && !unit && !transT) return LaunchTrsm<CT, true, false, false>(K, N, T, ldt, B, ldb, stream);
-    if (lower && !unit && transT) return LaunchTrsm<CT, true, false, true>(K, N, T, ldt, B, ldb, stream);
-    if (!lower && unit && !transT) return LaunchTrsm<CT, false, true, false>(K, N, T, ldt, B, ldb, stream);
-    if (!lower && unit && transT) return LaunchTrsm<CT, false, true, true>(K, N, T, ldt, B, ldb, stream);
-    if (!lower && !unit && !transT) return LaunchTrsm<CT, false, false, false>(K, N, T, ldt, B, ldb, stream);
-    return LaunchTrsm<CT, false, false, true>(K, N, T, ldt, B, ldb, stream);
 }
 
 }  // namespace
@@ -535,9 +523,32 @@ hipError_t TrsmBlock(int K, int N, const float* T, int ldt, bool lower, bool uni
                      hipStream_t stream)
 {
     if (K <= 0 || N <= 0) return hipSuccess;
-    if (K > 512) return hipErrorInvalidValue;
-    if (K <= 256) return TrsmDispatch<32>(K, N, T, ldt, lower, unit, transT, B, ldb, stream);
-    return TrsmDispatch<16>(K, N, T, ldt, lower, unit, transT, B, ldb, stream);
+    // op(T) is lower-triangular (forward substitution) when lower != transT.
+    const bool fwd = lower != transT;
+    const int nblk = (K + 63) / 64;
+    for (int q = 0; q < nblk; ++q)
+    {
+        const int bi = fwd ? q : nblk - 1 - q;
+        const int k0 = bi * 64, kb = std::min(64, K - k0), k1 = k0 + kb;
+        const float* Tkk = T + static_cast<size_t>(k0) * ldt + k0;
+        hipError_t e = TrsmDiag(kb, N, Tkk, ldt, fwd, unit, transT, B + k0, ldb, stream);
+        if (e != hipSuccess) return e;
+        // Fold the solved rows X = B[k0:k1) into the rows still to be solved.
+        if (fwd && k1 < K)
+        {
+            // rows [k1, K) of op(T), columns [k0, k1)
+            const float* Ablk = transT ? T + static_cast<size_t>(k1) * ldt + k0 : T + static_cast<size_t>(k0) * ldt + k1;
+            e = GemmSub(K - k1, N, kb, Ablk, ldt, transT ? 1 : 0, B + k0, ldb, B + k1, ldb, Precision::FP32, stream);
+        }
+        else if (!fwd && k0 > 0)
+        {
+            // rows [0, k0) of op(T), columns [k0, k1)
+            const float* Ablk = transT ? T + k0 : T + static_cast<size_t>(k0) * ldt;
+            e = GemmSub(k0, N, kb, Ablk, ldt, transT ? 1 : 0, B + k0, ldb, B, ldb, Precision::FP32, stream);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace ops

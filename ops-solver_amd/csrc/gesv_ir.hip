@@ -6,21 +6,28 @@
 // host niter, device dinfo.
 //
 // Algorithm (per right-hand side):
-//   LU = P A with the trailing updates on MFMA in FP16 (per-tile power-of-two scaling) or
-//   BF16, FP32 accumulation and FP32 storage of the factors;
-//   x = (LU)^-1 b;
+//   factors: U^T A V = L U without pivoting after two levels of random butterflies (FP16 /
+//   BF16 forms, n % 4 == 0), or P A = L U with partial pivoting (FP32, other orders, and
+//   the fallback); trailing updates on MFMA in FP16 (per-tile power-of-two scaling), BF16
+//   or split BF16, FP32 accumulation, FP32 storage of the factors;
+//   x = M^-1 b;
 //   repeat: r = b - A x in FP64 (A read in FP32); stop when
 //     ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol;
-//     d = GMRES(M^-1 A, M^-1 r) with M = LU applied in FP32 and the Krylov basis, the
-//     Hessenberg least squares and the update in FP64;  x += d.
+//     d = GMRES(M^-1 A, M^-1 r) with M^-1 applied by FP64 substitution on the FP32
+//     factors and the Krylov basis, the Hessenberg least squares and the update in FP64;
+//     x += d.
 // Every reduction runs in a fixed order, so results are bitwise repeatable.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "lu_kernels.h"
+#include "trsv.h"
 
 namespace ops
 {
@@ -133,23 +140,12 @@ __global__ void scale_copy_kernel(int n, const double* __restrict__ src, const d
     dst[i] = invert ? (s != 0.0 ? src[i] / s : 0.0) : src[i] * s;
 }
 
-// y32 = (float)(v[perm[i]] / s), s = max|v| (scaling keeps the FP32 solve in range)
-__global__ void prep_rhs_kernel(int n, const double* __restrict__ v, const int* __restrict__ perm,
-                                const double* __restrict__ scal, int slot, float* __restrict__ y)
+// out[i] = v[perm[i]] (row interchanges of the pivoted factorization)
+__global__ void gather_f64_kernel(int n, const double* __restrict__ v, const int* __restrict__ perm,
+                                  double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double s = scal[slot] > 0.0 ? scal[slot] : 1.0;
-    y[i] = static_cast<float>(v[perm ? perm[i] : i] / s);
-}
-
-__global__ void finish_solve_kernel(int n, const float* __restrict__ y, const double* __restrict__ scal, int slot,
-                                    double* __restrict__ out)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double s = scal[slot] > 0.0 ? scal[slot] : 1.0;
-    out[i] = static_cast<double>(y[i]) * s;
+    if (i < n) out[i] = v[perm[i]];
 }
 
 // x += V(:, 0:k) * y
@@ -196,9 +192,9 @@ struct IrWork
     float* lu;
     int* perm;
     void* getrfWork;
-    float* y32;
     double *x, *b, *r, *w, *V, *partial, *scal, *small, *t64;
     float *du, *dv;  // random butterfly diagonals (2n each)
+    void* trsv;      // chained triangular-solve state
     size_t bytes;
 };
 
@@ -219,7 +215,6 @@ IrWork IrLayout(void* base, int n, int restart)
     w.lu = reinterpret_cast<float*>(take(static_cast<size_t>(n) * n * sizeof(float)));
     w.perm = reinterpret_cast<int*>(take(static_cast<size_t>(n) * sizeof(int)));
     w.getrfWork = take(RealGetrfWorkspaceBytes(n, n));
-    w.y32 = reinterpret_cast<float*>(take(static_cast<size_t>(n) * sizeof(float)));
     w.x = reinterpret_cast<double*>(take(static_cast<size_t>(n) * sizeof(double)));
     w.b = reinterpret_cast<double*>(take(static_cast<size_t>(n) * sizeof(double)));
     w.r = reinterpret_cast<double*>(take(static_cast<size_t>(n) * sizeof(double)));
@@ -231,6 +226,7 @@ IrWork IrLayout(void* base, int n, int restart)
     w.t64 = reinterpret_cast<double*>(take(static_cast<size_t>(n) * sizeof(double)));
     w.du = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
     w.dv = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
+    w.trsv = take(TrsvChainStateBytes(n));
     w.bytes = off;
     return w;
 }
@@ -243,8 +239,7 @@ enum
     S_XNORM = 2,
     S_BNORM = 3,
     S_BETA = 4,
-    S_HNEXT = 5,
-    S_VSCALE = 6
+    S_HNEXT = 5
 };
 
 }  // namespace
@@ -259,6 +254,11 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
                   hipStream_t stream)
 {
     IrWork w = IrLayout(work, n, restart);
+    // OPS_SOLVER_TRACE=1 prints one line per refinement step to stderr (the loop already
+    // synchronizes there, so tracing adds no extra waits).
+    static const bool trace = std::getenv("OPS_SOLVER_TRACE") != nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
     const int chunks = (n + CB - 1) / CB;
     // The mixed precisions factor U^T A V without pivoting (random butterflies, n % 4 == 0);
     // FP32, other orders and the fallback after a refinement failure use partial pivoting.
@@ -277,30 +277,35 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
         hipLaunchKernelGGL(copy_cm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, stream, n, n, A, lda, w.lu, n);
         return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream);
     };
+    TrsvChainState ts = TrsvChainLayout(w.trsv, n);
+    if ((e = TrsvChainReset(ts, stream)) != hipSuccess) return e;
     if ((e = factor()) != hipSuccess) return e;
-    const dim3 gemvGrid((n + RB - 1) / RB, chunks);
-    auto solveM = [&](const double* v, double* out, int slot) -> hipError_t
+    if (trace)
     {
-        // out = M^-1 v, M = P^T L U (pivoted) or U^-T L U V^-1 (butterflies), with the LU
-        // solves in FP32 on the max-scaled vector.
-        const double* src = v;
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        std::fprintf(stderr, "[gesv-ir] n=%d factor (%s, precision %d) t=%.2f ms\n", n, useRbt ? "rbt" : "pivoted",
+                     static_cast<int>(prec), elapsed());
+        std::fflush(stderr);
+    }
+    const dim3 gemvGrid((n + RB - 1) / RB, chunks);
+    auto solveM = [&](const double* v, double* out) -> hipError_t
+    {
+        // out = M^-1 v, M = P^T L U (pivoted) or U^-T L U V^-1 (butterflies). The two
+        // substitutions run in FP64 on the FP32 factors: GMRES-IR only converges at large
+        // condition numbers when the preconditioned operator is applied at a precision
+        // beyond that of the factors (Carson & Higham, SIAM J. Sci. Comput. 40(2), 2018).
+        double* t = useRbt ? w.t64 : out;
+        hipError_t err;
         if (useRbt)
-        {
-            hipError_t err = RbtApplyVecLeft(n, v, w.t64, w.du, stream);
-            if (err != hipSuccess) return err;
-            src = w.t64;
-        }
-        hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, src, w.scal, slot);
-        hipLaunchKernelGGL(prep_rhs_kernel, G1(n), dim3(256), 0, stream, n, src, useRbt ? nullptr : w.perm, w.scal,
-                           slot, w.y32);
-        hipError_t err = RealLuSolveInPlace(0, n, 1, w.lu, n, w.y32, n, stream);
-        if (useRbt)
-        {
-            hipLaunchKernelGGL(finish_solve_kernel, G1(n), dim3(256), 0, stream, n, w.y32, w.scal, slot, w.t64);
-            if (err == hipSuccess) err = RbtApplyVecRight(n, w.t64, out, w.dv, stream);
-        }
+            err = RbtApplyVecLeft(n, v, t, w.du, stream);
         else
-            hipLaunchKernelGGL(finish_solve_kernel, G1(n), dim3(256), 0, stream, n, w.y32, w.scal, slot, out);
+        {
+            hipLaunchKernelGGL(gather_f64_kernel, G1(n), dim3(256), 0, stream, n, v, w.perm, t);
+            err = hipGetLastError();
+        }
+        if (err == hipSuccess) err = TrsvChain(ts, false, w.lu, n, t, stream);
+        if (err == hipSuccess) err = TrsvChain(ts, true, w.lu, n, t, stream);
+        if (err == hipSuccess && useRbt) err = RbtApplyVecRight(n, t, out, w.dv, stream);
         return err;
     };
     auto residual = [&](const double* x, double* r)
@@ -328,7 +333,7 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
                            w.b);
         hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.b, w.scal, S_BNORM);
     restart_rhs:
-        if ((e = solveM(w.b, w.x, S_VSCALE)) != hipSuccess) return e;
+        if ((e = solveM(w.b, w.x)) != hipSuccess) return e;
         bool done = prec == Precision::FP32;
         int outer = 0;
         if (useRbt)
@@ -350,11 +355,20 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
             residual(w.x, w.r);
             hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.r, w.scal, S_RNORM);
             hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.x, w.scal, S_XNORM);
+            int chainErr = 0;
             if ((e = hipMemcpyAsync(hscal.data(), w.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
                     hipSuccess ||
+                (e = hipMemcpyAsync(&chainErr, ts.err, sizeof(int), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
                 (e = hipStreamSynchronize(stream)) != hipSuccess)
                 return e;
+            if (chainErr != 0) return hipErrorLaunchTimeOut;
             const double berr = hscal[S_RNORM] / (hscal[S_ANORM] * hscal[S_XNORM] + hscal[S_BNORM]);
+            if (trace)
+            {
+                std::fprintf(stderr, "[gesv-ir] n=%d rhs=%d %s outer=%d berr=%.3e gmres=%d t=%.2f ms\n", n, c,
+                             useRbt ? "rbt" : "pivoted", outer, berr, totalIters, elapsed());
+                std::fflush(stderr);
+            }
             if (!(berr > tol)) break;
             if (outer++ >= maxOuter)
             {
@@ -368,7 +382,7 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
                 break;
             }
             // GMRES(restart) on M^-1 A d = M^-1 r, d0 = 0.
-            if ((e = solveM(w.r, w.w, S_VSCALE)) != hipSuccess) return e;
+            if ((e = solveM(w.r, w.w)) != hipSuccess) return e;
             hipLaunchKernelGGL(reduce_kernel<1>, dim3(1), dim3(1024), 0, stream, n, w.w, w.scal, S_BETA);
             hipLaunchKernelGGL(scale_copy_kernel, G1(n), dim3(256), 0, stream, n, w.w, w.scal, S_BETA, true, w.V);
             if ((e = hipMemcpyAsync(hscal.data(), w.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
@@ -385,7 +399,7 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
             {
                 double* vj = w.V + static_cast<size_t>(j) * n;
                 matvec(vj, w.r);
-                if ((e = solveM(w.r, w.w, S_VSCALE)) != hipSuccess) return e;
+                if ((e = solveM(w.r, w.w)) != hipSuccess) return e;
                 // classical Gram-Schmidt, twice
                 double* hdev = w.small;
                 for (int pass = 0; pass < 2; ++pass)

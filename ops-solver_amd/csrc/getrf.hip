@@ -24,6 +24,7 @@
 #include <type_traits>
 
 #include "lu_kernels.h"
+#include "trsv.h"
 
 namespace ops
 {
@@ -522,9 +523,20 @@ __global__ void permute_rows_kernel(int n, int nrhs, const int* __restrict__ per
 int SolveBlock(int n) { return n >= 4096 ? 256 : 128; }
 }  // namespace
 
+namespace
+{
+size_t GetrsVectorBytes(int n, int nrhs)
+{
+    return ((static_cast<size_t>(n) * std::max(nrhs, 1) + static_cast<size_t>(n) + 64) * sizeof(float) + 255) / 256 *
+           256;
+}
+// One right-hand side of at least this order uses the single-launch chained solve.
+constexpr int kChainMinN = 512;
+}  // namespace
+
 size_t RealGetrsWorkspaceBytes(int n, int nrhs)
 {
-    return (static_cast<size_t>(n) * std::max(nrhs, 1) + static_cast<size_t>(n) + 64) * sizeof(float);
+    return GetrsVectorBytes(n, nrhs) + TrsvChainStateBytes(std::max(n, 1));
 }
 
 // Triangular solves on a column-major n x nrhs block X (ldx) with packed LU factors.
@@ -637,7 +649,15 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
     if (trans == 0)
     {
         RET_IF(PermuteRows(true, n, nrhs, p, b, ldb, x, n, stream));
-        RET_IF(RealLuSolveInPlace(0, n, nrhs, lu, lda, x, n, stream));
+        if (nrhs == 1 && n >= kChainMinN)
+        {
+            TrsvChainState ts = TrsvChainLayout(static_cast<char*>(work) + GetrsVectorBytes(n, nrhs), n);
+            RET_IF(TrsvChainReset(ts, stream));
+            RET_IF(TrsvChain(ts, false, lu, lda, x, stream));
+            RET_IF(TrsvChain(ts, true, lu, lda, x, stream));
+        }
+        else
+            RET_IF(RealLuSolveInPlace(0, n, nrhs, lu, lda, x, n, stream));
         // copy back
         RET_IF(PermuteRows(true, n, nrhs, nullptr, x, n, b, ldb, stream));
     }

@@ -205,7 +205,7 @@ def test_invalid_arguments(torch, handle):
 
 
 @pytest.mark.parametrize("trans", [0, 1])
-@pytest.mark.parametrize("n,nrhs", [(5, 3), (300, 7), (1100, 2)])
+@pytest.mark.parametrize("n,nrhs", [(5, 3), (300, 7), (1100, 2), (512, 1), (777, 1), (2049, 1)])
 def test_getrs_device_and_host(torch, handle, trans, n, nrhs):
     rng = np.random.default_rng(n + trans)
     a = (rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32)
