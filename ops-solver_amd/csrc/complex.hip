@@ -393,60 +393,56 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int M, int N, int K, con
     }
 }
 
-// B <- op(T)^{-1} B, complex, K <= 256. op: transT 0 = T, 1 = T^T, 2 = T^H.
-template <bool LOWER, bool UNIT, int TRANS>
-__global__ __launch_bounds__(256) void ctrsm_block_kernel(int K, int N, const float2* __restrict__ T, int ldt,
-                                                          float2* __restrict__ B, int ldb)
+// B <- op(T)^{-1} B on one diagonal block of at most 64 rows, complex; op: 0 = T, 1 = T^T,
+// 2 = T^H. Every thread owns one column of B in registers and runs the substitution on it
+// alone; the block of op(T) (and the FP64 reciprocals of its diagonal) sit in LDS and are
+// read by all lanes at the same address. Per element: the same SubMul step per term and
+// the same FP64-reciprocal scaling per pivot as the unblocked complex solve.
+template <bool FWD, bool UNIT, int TRANS>
+__global__ __launch_bounds__(256) void ctrsm_diag64_kernel(int kb, int N, const float2* __restrict__ T, int ldt,
+                                                           float2* __restrict__ B, int ldb)
 {
-    constexpr int CT = 16;
-    constexpr int G = 256 / CT;
-    extern __shared__ float2 sBc[];  // K x (CT + 1)
-    constexpr bool FWD = LOWER != (TRANS != 0);
-    const int tid = threadIdx.x, c = tid % CT, g = tid / CT;
-    const int colBase = blockIdx.x * CT;
-    const bool active = colBase + c < N;
-    float2* bcol = B + static_cast<size_t>(colBase + c) * ldb;
-    auto opT = [&](int i, int s) -> float2
+    __shared__ float2 Ts[64][64];  // Ts[s][i] = op(T)(i, s)
+    __shared__ double2 Dinv[64];
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        float2 v = TRANS ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i];
-        return Conj(v, TRANS == 2);
-    };
-    auto diagInv = [&](int s) -> double2 { return Recip64(Conj(T[static_cast<size_t>(s) * ldt + s], TRANS == 2)); };
-    for (int i = g; i < K; i += G) sBc[i * (CT + 1) + c] = active ? bcol[i] : make_float2(0.0f, 0.0f);
+        const int s = e >> 6, i = e & 63;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (i < kb && s < kb)
+            v = Conj(TRANS ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i], TRANS == 2);
+        Ts[s][i] = v;
+    }
+    if (!UNIT)
+        for (int s = threadIdx.x; s < kb; s += 256) Dinv[s] = Recip64(Conj(T[static_cast<size_t>(s) * ldt + s], TRANS == 2));
     __syncthreads();
-    const int first = FWD ? 0 : K - 1;
-    if (!UNIT && g == 0 && active) sBc[first * (CT + 1) + c] = Mul64(sBc[first * (CT + 1) + c], diagInv(first));
-    __syncthreads();
-    if (FWD)
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= N) return;
+    float2* b = B + static_cast<size_t>(col) * ldb;
+    float2 x[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[i] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int t = 0; t < 64; ++t)
     {
-        for (int s = 0; s < K - 1; ++s)
+        const int s = FWD ? t : 63 - t;
+        if (s < kb)
         {
-            const float2 x = sBc[s * (CT + 1) + c];
-            for (int i = s + 1 + g; i < K; i += G)
+            if (!UNIT) x[s] = Mul64(x[s], Dinv[s]);
+            if (FWD)
             {
-                float2 v = SubMul(sBc[i * (CT + 1) + c], opT(i, s), x);
-                if (!UNIT && i == s + 1) v = Mul64(v, diagInv(i));
-                sBc[i * (CT + 1) + c] = v;
+#pragma unroll
+                for (int i = s + 1; i < 64; ++i) x[i] = SubMul(x[i], Ts[s][i], x[s]);
             }
-            __syncthreads();
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < s; ++i) x[i] = SubMul(x[i], Ts[s][i], x[s]);
+            }
         }
     }
-    else
-    {
-        for (int s = K - 1; s > 0; --s)
-        {
-            const float2 x = sBc[s * (CT + 1) + c];
-            for (int i = s - 1 - g; i >= 0; i -= G)
-            {
-                float2 v = SubMul(sBc[i * (CT + 1) + c], opT(i, s), x);
-                if (!UNIT && i == s - 1) v = Mul64(v, diagInv(i));
-                sBc[i * (CT + 1) + c] = v;
-            }
-            __syncthreads();
-        }
-    }
-    if (active)
-        for (int i = g; i < K; i += G) bcol[i] = sBc[i * (CT + 1) + c];
+#pragma unroll
+    for (int i = 0; i < 64; ++i)
+        if (i < kb) b[i] = x[i];
 }
 
 hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, const float2* B, int ldb, float2* C,
@@ -464,32 +460,63 @@ hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, c
     return hipGetLastError();
 }
 
-template <bool LOWER, bool UNIT, int TRANS>
-hipError_t LaunchCTrsm(int K, int N, const float2* T, int ldt, float2* B, int ldb, hipStream_t stream)
+template <bool FWD, bool UNIT, int TRANS>
+hipError_t LaunchCTrsmDiag(int kb, int N, const float2* T, int ldt, float2* B, int ldb, hipStream_t stream)
 {
-    const size_t shmem = static_cast<size_t>(K) * 17 * sizeof(float2);
-    hipLaunchKernelGGL((ctrsm_block_kernel<LOWER, UNIT, TRANS>), dim3((N + 15) / 16), dim3(256), shmem, stream, K, N,
-                       T, ldt, B, ldb);
+    hipLaunchKernelGGL((ctrsm_diag64_kernel<FWD, UNIT, TRANS>), dim3((N + 255) / 256), dim3(256), 0, stream, kb, N, T,
+                       ldt, B, ldb);
     return hipGetLastError();
 }
 
+hipError_t CTrsmDiag(int kb, int N, const float2* T, int ldt, bool fwd, bool unit, int trans, float2* B, int ldb,
+                     hipStream_t stream)
+{
+#define CD_CASE(F, U, TR) \
+    if (fwd == F && unit == U && trans == TR) return LaunchCTrsmDiag<F, U, TR>(kb, N, T, ldt, B, ldb, stream)
+    CD_CASE(true, true, 0);
+    CD_CASE(true, false, 0);
+    CD_CASE(false, true, 0);
+    CD_CASE(false, false, 0);
+    CD_CASE(true, true, 1);
+    CD_CASE(true, false, 1);
+    CD_CASE(false, true, 1);
+    CD_CASE(false, false, 1);
+    CD_CASE(true, true, 2);
+    CD_CASE(true, false, 2);
+    CD_CASE(false, true, 2);
+    CD_CASE(false, false, 2);
+#undef CD_CASE
+    return hipErrorInvalidValue;
+}
+
+// B <- op(T)^{-1} B, complex, any K: 64-row diagonal solves interleaved with complex GEMM
+// updates of the rows still to be solved.
 hipError_t CTrsmBlock(int K, int N, const float2* T, int ldt, bool lower, bool unit, int trans, float2* B, int ldb,
                       hipStream_t stream)
 {
     if (K <= 0 || N <= 0) return hipSuccess;
-    if (K > 256) return hipErrorInvalidValue;
-#define CT_CASE(L, U, TR) \
-    if (lower == L && unit == U && trans == TR) return LaunchCTrsm<L, U, TR>(K, N, T, ldt, B, ldb, stream)
-    CT_CASE(true, true, 0);
-    CT_CASE(true, true, 1);
-    CT_CASE(true, true, 2);
-    CT_CASE(false, false, 0);
-    CT_CASE(false, false, 1);
-    CT_CASE(false, false, 2);
-    CT_CASE(true, false, 0);
-    CT_CASE(false, true, 0);
-#undef CT_CASE
-    return hipErrorInvalidValue;
+    const bool fwd = lower != (trans != 0);
+    const int nblk = (K + 63) / 64;
+    for (int q = 0; q < nblk; ++q)
+    {
+        const int bi = fwd ? q : nblk - 1 - q;
+        const int k0 = bi * 64, kb = std::min(64, K - k0), k1 = k0 + kb;
+        hipError_t e = CTrsmDiag(kb, N, T + static_cast<size_t>(k0) * ldt + k0, ldt, fwd, unit, trans, B + k0, ldb,
+                                 stream);
+        if (e != hipSuccess) return e;
+        if (fwd && k1 < K)
+        {
+            const float2* Ablk = trans ? T + static_cast<size_t>(k1) * ldt + k0 : T + static_cast<size_t>(k0) * ldt + k1;
+            e = CGemmSub(K - k1, N, kb, Ablk, ldt, trans, B + k0, ldb, B + k1, ldb, stream);
+        }
+        else if (!fwd && k0 > 0)
+        {
+            const float2* Ablk = trans ? T + k0 : T + static_cast<size_t>(k0) * ldt;
+            e = CGemmSub(k0, N, kb, Ablk, ldt, trans, B + k0, ldb, B, ldb, stream);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 struct CWork

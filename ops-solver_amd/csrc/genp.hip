@@ -278,6 +278,8 @@ hipError_t RbtApplyVecLeft(int n, const double* x, double* y, const float* du, h
     return hipGetLastError();
 }
 
+int GenpOuterBlock() { return OB; }
+
 hipError_t RbtApplyVecRight(int n, const double* x, double* y, const float* dv, hipStream_t s)
 {
     hipLaunchKernelGGL(rbt_vec_right_kernel, dim3((n / 4 + 255) / 256), dim3(256), 0, s, n, x, y, dv);
@@ -285,8 +287,9 @@ hipError_t RbtApplyVecRight(int n, const double* x, double* y, const float* dv, 
 }
 
 // LU without pivoting of an n x n matrix in place. Trailing updates of depth OB run with
-// the requested MFMA input precision; the in-panel updates run in FP32.
-hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing, hipStream_t s)
+// the requested MFMA input precision (on packed operands when `pack` holds
+// GemmPackedWorkspaceBytes(n, n, OB)); the in-panel updates run in FP32.
+hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing, void* pack, hipStream_t s)
 {
     FillInfo(info, 1, 0, s);
     for (int J = 0; J < n; J += OB)
@@ -319,9 +322,14 @@ hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing,
             GRET(TrsmBlock(jb, right, a + static_cast<size_t>(J) * lda + J, lda, true, true, false,
                            a + static_cast<size_t>(Jend) * lda + J, lda, s));
             ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * right * right * jb, s);
-            GRET(GemmSub(right, right, jb, a + static_cast<size_t>(J) * lda + Jend, lda, 0,
-                         a + static_cast<size_t>(Jend) * lda + J, lda, a + static_cast<size_t>(Jend) * lda + Jend, lda,
-                         trailing, s));
+            if (pack != nullptr)
+                GRET(GemmSubPacked(right, right, jb, a + static_cast<size_t>(J) * lda + Jend, lda,
+                                   a + static_cast<size_t>(Jend) * lda + J, lda,
+                                   a + static_cast<size_t>(Jend) * lda + Jend, lda, trailing, pack, s));
+            else
+                GRET(GemmSub(right, right, jb, a + static_cast<size_t>(J) * lda + Jend, lda, 0,
+                             a + static_cast<size_t>(Jend) * lda + J, lda, a + static_cast<size_t>(Jend) * lda + Jend,
+                             lda, trailing, s));
         }
     }
     return hipGetLastError();

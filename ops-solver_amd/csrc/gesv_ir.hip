@@ -32,7 +32,7 @@
 namespace ops
 {
 hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream);
+                         int* permOut, hipStream_t stream, void* pack);
 hipError_t RealLuSolveInPlace(int trans, int n, int nrhs, const float* lu, int lda, float* x, int ldx,
                               hipStream_t stream);
 hipError_t RbtMakeDiagonals(int n, unsigned long long seed, float* du, float* dv, hipStream_t s);
@@ -40,7 +40,8 @@ hipError_t RbtTransform(int n, const float* A, int lda, float* dst, int ldd, con
                         hipStream_t s);
 hipError_t RbtApplyVecLeft(int n, const double* x, double* y, const float* du, hipStream_t s);
 hipError_t RbtApplyVecRight(int n, const double* x, double* y, const float* dv, hipStream_t s);
-hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing, hipStream_t s);
+hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing, void* pack, hipStream_t s);
+int GenpOuterBlock();
 
 namespace
 {
@@ -195,6 +196,7 @@ struct IrWork
     double *x, *b, *r, *w, *V, *partial, *scal, *small, *t64;
     float *du, *dv;  // random butterfly diagonals (2n each)
     void* trsv;      // chained triangular-solve state
+    void* pack;      // packed BF16 operands of the trailing update
     size_t bytes;
 };
 
@@ -227,6 +229,7 @@ IrWork IrLayout(void* base, int n, int restart)
     w.du = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
     w.dv = reinterpret_cast<float*>(take(static_cast<size_t>(2 * n) * sizeof(float)));
     w.trsv = take(TrsvChainStateBytes(n));
+    w.pack = take(GemmPackedWorkspaceBytes(n, n, std::max(512, GenpOuterBlock())));
     w.bytes = off;
     return w;
 }
@@ -270,12 +273,12 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
         {
             hipError_t err = RbtMakeDiagonals(n, 0x5EED0000ull + static_cast<unsigned long long>(n), w.du, w.dv, stream);
             if (err == hipSuccess) err = RbtTransform(n, A, lda, w.lu, n, w.du, w.dv, stream);
-            if (err == hipSuccess) err = GetrfNoPivot(n, w.lu, n, info, prec, stream);
+            if (err == hipSuccess) err = GetrfNoPivot(n, w.lu, n, info, prec, w.pack, stream);
             hipLaunchKernelGGL(identity_ipiv_kernel, G1(n), dim3(256), 0, stream, n, ipiv);
             return err == hipSuccess ? hipGetLastError() : err;
         }
         hipLaunchKernelGGL(copy_cm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, stream, n, n, A, lda, w.lu, n);
-        return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream);
+        return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream, w.pack);
     };
     TrsvChainState ts = TrsvChainLayout(w.trsv, n);
     if ((e = TrsvChainReset(ts, stream)) != hipSuccess) return e;

@@ -5,14 +5,15 @@
 // panel, swap+TRSM, trailing product per block). The MI355X design:
 //   * panel: a sequence of register-resident leaf factorizations (leaf_getf2_kernel), one
 //     workgroup of 256/1024 threads holding every active row of W panel columns in VGPRs.
-//     Per column the pivot is a wave64 argmax (DPP/cross-lane shuffles) followed by a
-//     16-entry LDS reduction; the row swap, the scaling and the rank-1 update never leave
-//     the CU. Pivot rule, zero-pivot info and reciprocal scaling follow
+//     Per column the pivot is a 64-bit (|a|, position) key argmax: DPP within the wave,
+//     then the wave winners through LDS with one barrier; the row swap, the scaling and
+//     the rank-1 update never leave the CU. Pivot rule, zero-pivot info and reciprocal scaling follow
 //     src/sgetrf/real_lu_blocked_kernel.cpp:342-389, :467-482 and
 //     src/utils/kernel/real_lu.hpp:419-437.
-//   * within the panel, each leaf's row interchanges are applied to the other panel columns
-//     as a gather/scatter of the (<= 2W) moved rows, followed by the W x W unit-lower solve
-//     and an MFMA update of the rest of the panel.
+//   * the panel is halved recursively down to leaves; each leaf applies its row
+//     interchanges to the other panel columns itself (a gather/scatter of the <= 2W moved
+//     rows), and every split solves the right half's top rows with the left half's L11
+//     and updates the rest of the right half with one MFMA product.
 //   * after the panel, the composed interchanges of all NB pivots are applied to every
 //     column outside the panel (one coalesced gather/scatter per column), then the NB x NB
 //     unit-lower TRSM on the U12 block and the MFMA trailing update A22 -= L21 * U12.
@@ -55,26 +56,67 @@ inline Work Layout(void* base, int m)
 }
 inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 4 * NBMAX + 8 + 1 + 2 * 128 + 7; }
 
+// Pivot key of a candidate: larger |a| first, then the lower position. The high word is
+// |a|'s bit pattern plus one (non-negative floats order like their bits) and 0 for NaN, so
+// a NaN never wins over a number and an all-NaN column keeps its diagonal position, as
+// with the comparison |a| > best; the low word is ~position.
+__device__ inline unsigned long long PivotKey(float v, int pos)
+{
+    const float a = fabsf(v);
+    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;
+    return (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~pos);
+}
+
+template <int CTRL>
+__device__ inline unsigned long long DppU64(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xffffffffull), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, 0xF, 0xF, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+
+__device__ inline unsigned long long ReadLaneU64(unsigned long long v, int lane)
+{
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Maximum over the wave64: quad swaps and row rotations (DPP) give every lane its 16-lane
+// row maximum, four lane reads combine the rows.
+__device__ inline unsigned long long WaveMaxU64(unsigned long long v)
+{
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    v = mx(v, DppU64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, DppU64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, DppU64<0x124>(v));  // row_ror:4
+    v = mx(v, DppU64<0x128>(v));  // row_ror:8
+    return mx(mx(ReadLaneU64(v, 0), ReadLaneU64(v, 16)), mx(ReadLaneU64(v, 32), ReadLaneU64(v, 48)));
+}
+
 // ---------------------------------------------------------------------------------------
-// Leaf: factor columns [c0, c0+wact) over rows [c0, m) held in registers.
-// Thread t owns local rows t, t+T, t+2T, ... (R of them). Rows never move between
-// registers: each row tracks the position it occupies under the LAPACK interchange
-// sequence, the pivot of column j is the candidate (position >= j) with the largest |a|
-// and, on ties, the lowest position -- exactly the first-row rule of a physical swap.
-// At the end each row is stored at its final position, which applies the interchanges
-// to the leaf columns.
+// Leaf factorization: columns [c0, c0 + wact) of rows [c0, m), T threads, R rows per
+// thread (row tid + i*T), W >= wact columns in registers. Pivoting is implicit: each row
+// tracks the position it occupies under the LAPACK interchange sequence, the pivot of
+// column j is the candidate (position >= j) with the largest |a| and, on ties, the lowest
+// position -- the first-row rule of a physical swap. Per column: one 64-bit key argmax
+// per thread, a DPP wave argmax, and one barrier, after which every thread reads the
+// wave winners' keys and the chosen candidate's row from LDS (double-buffered by column
+// parity, so one barrier per column suffices). At the end each row is stored at its final
+// position, which applies the interchanges to the leaf columns.
 // ---------------------------------------------------------------------------------------
 template <int T, int R, int W>
 __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
-                                                       int pivoting, int newPanel, int* __restrict__ ipiv,
-                                                       int* __restrict__ info, int* __restrict__ rowsrc,
-                                                       int* __restrict__ leafmap)
+                                                       int panelEnd, int pivoting, int newPanel,
+                                                       int* __restrict__ ipiv, int* __restrict__ info,
+                                                       int* __restrict__ rowsrc)
 {
     constexpr int NW = T / 64;
-    __shared__ float s_prow[W];
-    __shared__ float s_best[NW];
-    __shared__ int s_bpos[NW];
+    constexpr int MOVES = 2 * W;  // a leaf moves at most 2 * wact rows
+    __shared__ unsigned long long s_key[2][NW];
+    __shared__ float s_row[2][NW][W];
     __shared__ int s_count;
+    __shared__ int s_mdst[MOVES], s_msrc[MOVES];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int mp = m - c0;
@@ -99,66 +141,50 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
     {
         if (j < wact)  // uniform
         {
-            int bpos = j;
-            if (pivoting)
-            {
-                float best = -1.0f;
-                int bp = INT_MAX;
-#pragma unroll
-                for (int i = 0; i < R; ++i)
-                {
-                    if (pos[i] >= j && pos[i] != INT_MAX)
-                    {
-                        const float v = fabsf(a[i][j]);
-                        if (v > best || (v == best && pos[i] < bp))
-                        {
-                            best = v;
-                            bp = pos[i];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                {
-                    const float ob = __shfl_xor(best, off);
-                    const int op = __shfl_xor(bp, off);
-                    if (ob > best || (ob == best && op < bp))
-                    {
-                        best = ob;
-                        bp = op;
-                    }
-                }
-                if (lane == 0)
-                {
-                    s_best[wid] = best;
-                    s_bpos[wid] = bp;
-                }
-                __syncthreads();
-                best = s_best[0];
-                bp = s_bpos[0];
-#pragma unroll
-                for (int w = 1; w < NW; ++w)
-                {
-                    const float ob = s_best[w];
-                    const int op = s_bpos[w];
-                    if (ob > best || (ob == best && op < bp))
-                    {
-                        best = ob;
-                        bp = op;
-                    }
-                }
-                bpos = bp == INT_MAX ? j : bp;
-            }
-            // The pivot row publishes itself; positions j and bpos are exchanged.
+            const int par = j & 1;
+            // Candidate of this thread: its best row (or, without pivoting, the row at j).
+            unsigned long long best = 0ull;
+            int bi = -1;
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
-                if (pos[i] == bpos)
+                const bool cand = pivoting ? (pos[i] >= j && pos[i] != INT_MAX) : pos[i] == j;
+                const unsigned long long k = cand ? PivotKey(a[i][j], pos[i]) | (1ull << 63) : 0ull;
+                if (k > best)
                 {
-#pragma unroll
-                    for (int k = 0; k < W; ++k) s_prow[k] = a[i][k];
+                    best = k;
+                    bi = i;
                 }
             }
+            const unsigned long long wmax = WaveMaxU64(best);
+            if (best == wmax && bi >= 0)  // the wave's winner (keys are unique per position)
+            {
+                s_key[par][wid] = wmax;
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (i == bi)
+#pragma unroll
+                        for (int k = 0; k < W; ++k) s_row[par][wid][k] = a[i][k];
+            }
+            if (lane == 0 && wmax == 0ull) s_key[par][wid] = 0ull;
+            __syncthreads();
+            unsigned long long top = s_key[par][0];
+            int tw = 0;
+#pragma unroll
+            for (int w = 1; w < NW; ++w)
+            {
+                const unsigned long long kw = s_key[par][w];
+                if (kw > top)
+                {
+                    top = kw;
+                    tw = w;
+                }
+            }
+            const int bpos = top == 0ull ? j : static_cast<int>(~static_cast<unsigned>(top & 0xffffffffull));
+            float prow[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) prow[k] = s_row[par][tw][k];
+            // Positions j and bpos are exchanged.
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
@@ -168,8 +194,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                     pos[i] = bpos;
             }
             if (tid == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
-            __syncthreads();
-            const float d = s_prow[j];
+            const float d = prow[j];
             if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
             // Branch-free update: rows that are not below the pivot take l = 0, for which
             // the FMA leaves them unchanged (a zero pivot skips the column likewise).
@@ -183,9 +208,8 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                 const float l = below ? q : 0.0f;
                 a[i][j] = below ? q : a[i][j];
 #pragma unroll
-                for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, s_prow[k], a[i][k]);
+                for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, prow[k], a[i][k]);
             }
-            __syncthreads();
         }
     }
 
@@ -203,8 +227,8 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
             if (dst != lr)
             {
                 const int slot = atomicAdd(&s_count, 1);
-                leafmap[1 + 2 * slot] = c0 + dst;
-                leafmap[2 + 2 * slot] = c0 + lr;
+                s_mdst[slot] = c0 + dst;
+                s_msrc[slot] = c0 + lr;
             }
         }
     }
@@ -223,46 +247,31 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
         const int lr = tid + i * T;
         if (lr < mp) rowsrc[c0 + pos[i]] = srcv[i];
     }
-    if (tid == 0)
-    {
-        leafmap[0] = s_count;
-        if (zeroCol != 0 && *info == 0) *info = zeroCol;
-    }
-}
+    if (tid == 0 && zeroCol != 0 && *info == 0) *info = zeroCol;
 
-// Applies the leaf's moved-row map to panel columns outside the leaf and, for the panel
-// columns right of the leaf, solves the leaf's W x W unit-lower block on the leaf rows.
-// One workgroup (128 threads) per column.
-__global__ __launch_bounds__(128) void panel_swap_trsm_kernel(float* __restrict__ A, int lda, int j0, int c0,
-                                                              int wact, int panelEnd, const int* __restrict__ leafmap)
-{
-    __shared__ float s_x[64];
-    const int left = c0 - j0;
-    const int idx = blockIdx.x;
-    const int c = idx < left ? j0 + idx : c0 + wact + (idx - left);
-    float* col = A + static_cast<size_t>(c) * lda;
-    const int cnt = leafmap[0];
-    const int t = threadIdx.x;
-    float v = 0.0f;
-    int dst = -1;
-    if (t < cnt)
+    // The same interchanges on the other panel columns [j0, c0) and [c0 + wact, panelEnd):
+    // thread t takes moved entry t % MOVES of column t / MOVES of each batch of T / MOVES
+    // columns; every entry of a batch is gathered before any is scattered (rows only
+    // exchange among themselves).
+    const int cnt = s_count;
+    const int left = c0 - j0, others = left + (panelEnd - c0 - wact);
+    constexpr int CPB = T / MOVES;  // columns per batch
+    const int e = tid % MOVES;
+    for (int b0 = 0; b0 < others; b0 += CPB)  // uniform trip count
     {
-        dst = leafmap[1 + 2 * t];
-        v = col[leafmap[2 + 2 * t]];
-    }
-    __syncthreads();
-    if (dst >= 0) col[dst] = v;
-    __syncthreads();
-    if (c < c0) return;  // left columns only take the interchanges
-    if (t < wact) s_x[t] = col[c0 + t];
-    __syncthreads();
-    const float* L = A + static_cast<size_t>(c0) * lda + c0;
-    for (int s = 0; s < wact - 1; ++s)
-    {
-        if (t > s && t < wact) s_x[t] = fmaf(-L[static_cast<size_t>(s) * lda + t], s_x[s], s_x[t]);
+        const int q = b0 + tid / MOVES;
+        const bool act = q < others && e < cnt;
+        float* col = nullptr;
+        float v = 0.0f;
+        if (act)
+        {
+            const int c = q < left ? j0 + q : c0 + wact + (q - left);
+            col = A + static_cast<size_t>(c) * lda;
+            v = col[s_msrc[e]];
+        }
         __syncthreads();
+        if (act) col[s_mdst[e]] = v;
     }
-    if (t < wact) col[c0 + t] = s_x[t];
 }
 
 // Builds the list of rows moved by the whole panel (rowsrc[r] != r) and applies it to the
@@ -346,11 +355,11 @@ __global__ void iota_kernel(int* p, int n)
 }
 
 template <int T, int R, int W>
-hipError_t LaunchLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pivoting, int newPanel, int* ipiv,
-                      int* info, const Work& w, hipStream_t stream)
+hipError_t LaunchLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
+                      int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
-    hipLaunchKernelGGL((leaf_getf2_kernel<T, R, W>), dim3(1), dim3(T), 0, stream, A, lda, m, j0, c0, wact, pivoting,
-                       newPanel, ipiv, info, w.rowsrc, w.leafmap);
+    hipLaunchKernelGGL((leaf_getf2_kernel<T, R, W>), dim3(1), dim3(T), 0, stream, A, lda, m, j0, c0, wact, panelEnd,
+                       pivoting, newPanel, ipiv, info, w.rowsrc);
     return hipGetLastError();
 }
 
@@ -364,26 +373,20 @@ int LeafWidth(int mp)
     return 4;
 }
 
-hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int pivoting, int newPanel, int* ipiv, int* info,
-                const Work& w, hipStream_t stream)
+hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
+                int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
     const int mp = m - c0;
-#define LEAF(T, R, W) LaunchLeaf<T, R, W>(A, lda, m, j0, c0, wact, pivoting, newPanel, ipiv, info, w, stream)
+#define LEAF(T, R, W) \
+    LaunchLeaf<T, R, W>(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream)
     if (mp <= 256) return LEAF(256, 1, 32);
     if (mp <= 512) return LEAF(512, 1, 32);
     if (mp <= 1024) return LEAF(512, 2, 16);
     if (mp <= 2048) return LEAF(512, 4, 8);
     if (mp <= 4096) return LEAF(512, 8, 8);
     if (mp <= 8192) return LEAF(512, 16, 8);
-    return LEAF(512, 32, 4);
+    return LEAF(1024, 16, 4);
 #undef LEAF
-}
-
-int OuterBlock(int m, int n, Precision prec)
-{
-    const int k = std::min(m, n);
-    if (prec != Precision::FP32) return k >= 4096 ? 512 : (k >= 1024 ? 256 : 128);
-    return k >= 2048 ? 256 : 128;
 }
 
 #define RET_IF(e)                          \
@@ -392,6 +395,35 @@ int OuterBlock(int m, int n, Precision prec)
         hipError_t _e = (e);               \
         if (_e != hipSuccess) return _e;   \
     } while (0)
+
+// Factors columns [c0, c0 + w) of the outer panel [j0, panelEnd) over rows [c0, m); every
+// panel column already holds the updates of the columns left of c0. The left half is
+// factored first (recursively), its L11 solves the right half's top rows and an MFMA
+// product updates the rest of the right half, then the right half is factored. Each leaf
+// applies its row interchanges to all other panel columns itself.
+hipError_t PanelRec(float* a, int lda, int m, int j0, int panelEnd, int c0, int w, int pivoting, int* ipiv, int* info,
+                    const Work& wk, hipStream_t stream)
+{
+    const int W = LeafWidth(m - c0);
+    if (w <= W) return Leaf(a, lda, m, j0, c0, w, panelEnd, pivoting, c0 == j0, ipiv, info, wk, stream);
+    const int h = (w / 2 + W - 1) / W * W;  // whole leaves on the left
+    RET_IF(PanelRec(a, lda, m, j0, panelEnd, c0, h, pivoting, ipiv, info, wk, stream));
+    const int r = w - h;
+    RET_IF(TrsmBlock(h, r, a + static_cast<size_t>(c0) * lda + c0, lda, true, true, false,
+                     a + static_cast<size_t>(c0 + h) * lda + c0, lda, stream));
+    if (m - c0 - h > 0)
+        RET_IF(GemmSub(m - c0 - h, r, h, a + static_cast<size_t>(c0) * lda + c0 + h, lda, 0,
+                       a + static_cast<size_t>(c0 + h) * lda + c0, lda, a + static_cast<size_t>(c0 + h) * lda + c0 + h,
+                       lda, Precision::FP32, stream));
+    return PanelRec(a, lda, m, j0, panelEnd, c0 + h, r, pivoting, ipiv, info, wk, stream);
+}
+
+int OuterBlock(int m, int n, Precision prec)
+{
+    const int k = std::min(m, n);
+    if (prec != Precision::FP32) return k >= 4096 ? 512 : (k >= 1024 ? 256 : 128);
+    return k >= 2048 ? 256 : 128;
+}
 
 }  // namespace
 
@@ -402,9 +434,10 @@ size_t RealGetrfWorkspaceBytes(int m, int n)
 }
 
 // ipiv == nullptr selects no pivoting. perm (optional, m ints) receives the row
-// permutation P with (P A)[i] = A[perm[i]].
+// permutation P with (P A)[i] = A[perm[i]]. pack (optional, BF16 / BF16X3 trailing
+// precisions): GemmPackedWorkspaceBytes(m, n, 512) bytes for the packed trailing update.
 hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream)
+                         int* permOut, hipStream_t stream, void* pack)
 {
     if (m > kMaxRows) return hipErrorInvalidValue;
     Work w = Layout(work, m);
@@ -419,31 +452,8 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     {
         const int nb = std::min(NB, kmax - j0);
         const int panelEnd = j0 + nb;
-        // Panel: leaves of width W with in-panel interchange, TRSM and update.
-        for (int c0 = j0; c0 < panelEnd;)
-        {
-            const int W = LeafWidth(m - c0);
-            const int wact = std::min(W, panelEnd - c0);
-            RET_IF(Leaf(a, lda, m, j0, c0, wact, pivoting, c0 == j0, ipiv, info, w, stream));
-            const int others = nb - wact;
-            if (others > 0)
-            {
-                if (pivoting)
-                    hipLaunchKernelGGL(panel_swap_trsm_kernel, dim3(others), dim3(128), 0, stream, a, lda, j0, c0,
-                                       wact, panelEnd, w.leafmap);
-                else if (panelEnd - c0 - wact > 0)
-                    RET_IF(TrsmBlock(wact, panelEnd - c0 - wact, a + static_cast<size_t>(c0) * lda + c0, lda, true,
-                                     true, false, a + static_cast<size_t>(c0 + wact) * lda + c0, lda, stream));
-                RET_IF(hipGetLastError());
-                const int right = panelEnd - c0 - wact;
-                if (right > 0 && m - c0 - wact > 0)
-                    RET_IF(GemmSub(m - c0 - wact, right, wact, a + static_cast<size_t>(c0) * lda + c0 + wact, lda, 0,
-                                   a + static_cast<size_t>(c0 + wact) * lda + c0, lda,
-                                   a + static_cast<size_t>(c0 + wact) * lda + c0 + wact, lda, Precision::FP32,
-                                   stream));
-            }
-            c0 += wact;
-        }
+        // Panel: recursive halving down to register-resident leaves.
+        RET_IF(PanelRec(a, lda, m, j0, panelEnd, j0, nb, pivoting, ipiv, info, w, stream));
         // Interchanges outside the panel, U12 solve, trailing update.
         if (pivoting)
         {
@@ -463,9 +473,15 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
             if (m - panelEnd > 0)
             {
                 ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * (m - panelEnd) * right * nb, stream);
-                RET_IF(GemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
-                               a + static_cast<size_t>(panelEnd) * lda + j0, lda,
-                               a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, stream));
+                if (pack != nullptr)
+                    RET_IF(GemmSubPacked(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda,
+                                         a + static_cast<size_t>(panelEnd) * lda + j0, lda,
+                                         a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, pack,
+                                         stream));
+                else
+                    RET_IF(GemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
+                                   a + static_cast<size_t>(panelEnd) * lda + j0, lda,
+                                   a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, stream));
             }
         }
     }
@@ -475,7 +491,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
 hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
                      hipStream_t stream)
 {
-    return RealGetrfPerm(m, n, a, lda, ipiv, info, work, trailing, nullptr, stream);
+    return RealGetrfPerm(m, n, a, lda, ipiv, info, work, trailing, nullptr, stream, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------

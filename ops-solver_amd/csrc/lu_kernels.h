@@ -14,6 +14,13 @@ namespace ops
 hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, const float* B, int ldb, float* C,
                    int ldc, Precision prec, hipStream_t stream);
 
+// C(MxN) -= A(MxK) * B(KxN) (column-major, no transposes) on BF16 MFMA from operands packed
+// once into fragment order in `work` (GemmPackedWorkspaceBytes): prec BF16 (hi*hi) or
+// BF16X3 (split form); other precisions go to GemmSub.
+size_t GemmPackedWorkspaceBytes(int M, int N, int K);
+hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
+                         Precision prec, void* work, hipStream_t stream);
+
 // Triangular solve with a K x K diagonal block T (column-major, ldt) on a K x N block B:
 // B <- op(T)^{-1} B. lower: T's lower triangle is used (else upper); unit: unit diagonal;
 // transT: op(T) = T^T. Runs as 64-row diagonal solves (one column per thread, in
