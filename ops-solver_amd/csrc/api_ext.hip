@@ -11,7 +11,7 @@ namespace ops
 size_t GesvIrWorkspaceBytes(int n, int restart);
 hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
                   void* work, Precision prec, double tol, int maxOuter, int restart, int* info, int* niterOut,
-                  hipStream_t stream);
+                  hipStream_t stream, const LookAhead* la);
 }  // namespace ops
 
 using namespace ops;
@@ -76,7 +76,7 @@ aclsolverStatus_t Gesv(aclsolverHandle_t handle, Precision prec, int n, int nrhs
         piv = static_cast<int*>(extra);
     }
     if (GesvIR(n, nrhs, dA, ldda, piv, dB, lddb, dX, lddx, dWork, prec, h->irTolerance, h->irMaxOuter, h->irRestart,
-               dinfo, niter, s) != hipSuccess)
+               dinfo, niter, s, HandleLookAhead(handle)) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }

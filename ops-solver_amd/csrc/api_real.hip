@@ -13,7 +13,7 @@
 namespace ops
 {
 hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream, void* pack);
+                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la);
 hipError_t MatinvBatched(int n, const float* const* Aarr, const float* Abase, int lda, int64_t strideA,
                          float* const* Carr, float* Cbase, int ldc, int64_t strideC, int* ipiv, int* info, int batch,
                          hipStream_t stream);
@@ -211,7 +211,8 @@ OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n
     bool finite = true;
     if (!AllFinite(handle, A, m, n, lda, 0, 1, &finite)) return ACLSOLVER_STATUS_EXECUTION_FAILED;
     if (!finite) return Invalid(handle, devInfo, 1, 3);
-    if (RealGetrfPerm(m, n, A, lda, devIpiv, devInfo, Workspace, Precision::FP32, nullptr, s, nullptr) != hipSuccess)
+    if (RealGetrfPerm(m, n, A, lda, devIpiv, devInfo, Workspace, Precision::FP32, nullptr, s, nullptr,
+                      HandleLookAhead(handle)) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }

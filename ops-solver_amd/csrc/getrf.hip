@@ -34,14 +34,15 @@ namespace
 
 constexpr int kMaxRows = 16384;  // largest m the register-resident leaf supports
 
-// Workspace layout (ints): [0, m) panel row sources; [m, m + 4*NBMAX + 8) moved-row list;
-// then the leaf map (1 + 2*128 ints); then the global permutation (m ints).
+// Workspace layout (ints): [0, m) panel row sources; two moved-row lists of 4*NBMAX + 8
+// (panel k uses list k % 2, so the look-ahead panel can build its list while the previous
+// one is still being applied); then the global permutation (m ints).
 constexpr int NBMAX = 512;
+constexpr int kMovedInts = 4 * NBMAX + 8;
 struct Work
 {
     int* rowsrc;
-    int* moved;
-    int* leafmap;
+    int* moved[2];
     int* perm;
 };
 inline Work Layout(void* base, int m)
@@ -49,12 +50,12 @@ inline Work Layout(void* base, int m)
     int* p = static_cast<int*>(base);
     Work w;
     w.rowsrc = p;
-    w.moved = p + m;
-    w.leafmap = w.moved + 4 * NBMAX + 8;
-    w.perm = w.leafmap + 1 + 2 * 128 + 7;
+    w.moved[0] = p + m;
+    w.moved[1] = w.moved[0] + kMovedInts;
+    w.perm = w.moved[1] + kMovedInts;
     return w;
 }
-inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 4 * NBMAX + 8 + 1 + 2 * 128 + 7; }
+inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
 
 // Pivot key of a candidate: larger |a| first, then the lower position. The high word is
 // |a|'s bit pattern plus one (non-negative floats order like their bits) and 0 for NaN, so
@@ -317,16 +318,16 @@ __global__ __launch_bounds__(1024) void compact_moves_kernel(int m, int j0, cons
     }
 }
 
-// Applies the panel's moved-row list to columns [0, j0) and [j0+nb, n): one wave64 per
-// column gathers every source row, then scatters.
-__global__ __launch_bounds__(256) void outer_swap_kernel(float* __restrict__ A, int lda, int j0, int nb, int n,
-                                                         const int* __restrict__ moved)
+// Applies the panel's moved-row list to the columns [lo1, hi1) and [lo2, hi2): one wave64
+// per column gathers every source row, then scatters.
+__global__ __launch_bounds__(256) void outer_swap_kernel(float* __restrict__ A, int lda, int lo1, int hi1, int lo2,
+                                                         int hi2, const int* __restrict__ moved)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int idx = blockIdx.x * 4 + wave;
-    const int outside = j0 + (n - j0 - nb);
-    if (idx >= outside) return;
-    const int c = idx < j0 ? idx : idx + nb;
+    const int n1 = hi1 - lo1;
+    if (idx >= n1 + (hi2 - lo2)) return;
+    const int c = idx < n1 ? lo1 + idx : lo2 + (idx - n1);
     float* col = A + static_cast<size_t>(c) * lda;
     const int cnt = moved[0];
     constexpr int MAXQ = (2 * NBMAX + 63) / 64;
@@ -437,7 +438,7 @@ size_t RealGetrfWorkspaceBytes(int m, int n)
 // permutation P with (P A)[i] = A[perm[i]]. pack (optional, BF16 / BF16X3 trailing
 // precisions): GemmPackedWorkspaceBytes(m, n, 512) bytes for the packed trailing update.
 hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream, void* pack)
+                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la)
 {
     if (m > kMaxRows) return hipErrorInvalidValue;
     Work w = Layout(work, m);
@@ -448,50 +449,81 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     const int pivoting = ipiv != nullptr;
     if (permOut) hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, permOut, m);
     const int NB = OuterBlock(m, n, trailing);
-    for (int j0 = 0; j0 < kmax; j0 += NB)
+    const int npanel = (kmax + NB - 1) / NB;
+
+    // Look-ahead: the caller's stream s0 carries the critical path (the next block column's
+    // update and its panel); the side stream s1 carries the rest of each trailing update.
+    const bool two = la != nullptr && la->side != nullptr && npanel > 1;
+    hipStream_t s0 = stream, s1 = two ? la->side : stream;
+    if (two)
     {
-        const int nb = std::min(NB, kmax - j0);
-        const int panelEnd = j0 + nb;
-        // Panel: recursive halving down to register-resident leaves.
-        RET_IF(PanelRec(a, lda, m, j0, panelEnd, j0, nb, pivoting, ipiv, info, w, stream));
-        // Interchanges outside the panel, U12 solve, trailing update.
+        RET_IF(hipEventRecord(la->fork, s0));
+        RET_IF(hipStreamWaitEvent(s1, la->fork, 0));
+    }
+
+    // Panel k: factor, then list its interchanges and fold them into perm.
+    auto panel = [&](int k) -> hipError_t
+    {
+        const int j0 = k * NB, nb = std::min(NB, kmax - j0);
+        RET_IF(PanelRec(a, lda, m, j0, j0 + nb, j0, nb, pivoting, ipiv, info, w, s0));
         if (pivoting)
-        {
-            hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, stream, m, j0, w.rowsrc, w.moved,
+            hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, s0, m, j0, w.rowsrc, w.moved[k & 1],
                                permOut);
-            const int outside = j0 + (n - panelEnd);
-            if (outside > 0)
-                hipLaunchKernelGGL(outer_swap_kernel, dim3((outside + 3) / 4), dim3(256), 0, stream, a, lda, j0, nb,
-                                   n, w.moved);
-            RET_IF(hipGetLastError());
-        }
-        const int right = n - panelEnd;
-        if (right > 0)
+        return hipGetLastError();
+    };
+    // Applies panel k to the columns [lo, hi) (interchanges, U12 solve, trailing product)
+    // and, when leftSwaps, its interchanges to the columns left of the panel.
+    auto update = [&](int k, int lo, int hi, bool leftSwaps, bool packed, hipStream_t st) -> hipError_t
+    {
+        const int j0 = k * NB, nb = std::min(NB, kmax - j0), pe = j0 + nb;
+        const int cols = hi - lo, left = leftSwaps ? j0 : 0;
+        if (pivoting && cols + left > 0)
+            hipLaunchKernelGGL(outer_swap_kernel, dim3((cols + left + 3) / 4), dim3(256), 0, st, a, lda, 0, left, lo,
+                               hi, w.moved[k & 1]);
+        RET_IF(hipGetLastError());
+        if (cols <= 0) return hipSuccess;
+        RET_IF(TrsmBlock(nb, cols, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, false,
+                         a + static_cast<size_t>(lo) * lda + j0, lda, st));
+        if (m - pe <= 0) return hipSuccess;
+        const float* L21 = a + static_cast<size_t>(j0) * lda + pe;
+        const float* U12 = a + static_cast<size_t>(lo) * lda + j0;
+        float* A22 = a + static_cast<size_t>(lo) * lda + pe;
+        if (packed && pack != nullptr)
+            return GemmSubPacked(m - pe, cols, nb, L21, lda, U12, lda, A22, lda, trailing, pack, st);
+        return GemmSub(m - pe, cols, nb, L21, lda, 0, U12, lda, A22, lda, trailing, st);
+    };
+
+    RET_IF(panel(0));
+    if (two) RET_IF(hipEventRecord(la->panel[0], s0));
+    for (int k = 0; k < npanel; ++k)
+    {
+        const int pe = std::min(kmax, (k + 1) * NB);
+        const int nbNext = k + 1 < npanel ? std::min(NB, kmax - pe) : 0;
+        // Bulk of the trailing update (s1): everything right of the next block column.
+        if (two) RET_IF(hipStreamWaitEvent(s1, la->panel[k & 1], 0));
         {
-            RET_IF(TrsmBlock(nb, right, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, false,
-                             a + static_cast<size_t>(panelEnd) * lda + j0, lda, stream));
-            if (m - panelEnd > 0)
-            {
-                ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * (m - panelEnd) * right * nb, stream);
-                if (pack != nullptr)
-                    RET_IF(GemmSubPacked(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda,
-                                         a + static_cast<size_t>(panelEnd) * lda + j0, lda,
-                                         a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, pack,
-                                         stream));
-                else
-                    RET_IF(GemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
-                                   a + static_cast<size_t>(panelEnd) * lda + j0, lda,
-                                   a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, trailing, stream));
-            }
+            const double work2 = 2.0 * std::max(m - pe, 0) * std::max(n - pe - nbNext, 0) * std::min(NB, kmax - k * NB);
+            ProfileScope prof(ProfileKernel::TrailingGemm, work2, s1);
+            RET_IF(update(k, pe + nbNext, n, true, true, s1));
+        }
+        if (two) RET_IF(hipEventRecord(la->bulk[k & 1], s1));
+        // Critical path (s0): the next block column, then its panel.
+        if (nbNext > 0)
+        {
+            if (two && k > 0) RET_IF(hipStreamWaitEvent(s0, la->bulk[(k - 1) & 1], 0));
+            RET_IF(update(k, pe, pe + nbNext, false, false, s0));
+            RET_IF(panel(k + 1));
+            if (two) RET_IF(hipEventRecord(la->panel[(k + 1) & 1], s0));
         }
     }
+    if (two) RET_IF(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
     return hipGetLastError();
 }
 
 hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
                      hipStream_t stream)
 {
-    return RealGetrfPerm(m, n, a, lda, ipiv, info, work, trailing, nullptr, stream, nullptr);
+    return RealGetrfPerm(m, n, a, lda, ipiv, info, work, trailing, nullptr, stream, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -12,6 +12,16 @@
 namespace ops
 {
 
+// Side stream and events for look-ahead factorizations: the trailing update of block
+// step k runs on `side` while the caller's stream factors panel k + 1.
+struct LookAhead
+{
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t panel[2] = {nullptr, nullptr};
+    hipEvent_t bulk[2] = {nullptr, nullptr};
+};
+
 // Solver handle: the stream plus per-handle options and a reusable scratch allocation.
 // Reference: src/utils/aclsolver_handle_internal.h:33-100 (stream + per-context LU scratch).
 struct Handle
@@ -29,7 +39,13 @@ struct Handle
     int* hflag = nullptr;     // pinned host mirror of dflag
     size_t scratchBytes = 0;
     int numCUs = 256;
+    LookAhead la;             // created on first use
+    bool laReady = false;
 };
+
+// The handle's look-ahead resources (created on first call, default stream priority);
+// nullptr when they cannot be created, in which case factorizations run on one stream.
+const LookAhead* HandleLookAhead(aclsolverHandle_t h);
 
 inline Handle* H(aclsolverHandle_t h) { return static_cast<Handle*>(h); }
 inline hipStream_t Stream(aclsolverHandle_t h) { return H(h)->stream; }

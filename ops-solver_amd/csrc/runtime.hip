@@ -188,10 +188,44 @@ OPS_API aclsolverStatus_t aclsolverCreate(aclsolverHandle_t* handle)
 }
 
 // Reference :199-232: drains the stream before releasing handle-owned scratch.
+namespace ops
+{
+const LookAhead* HandleLookAhead(aclsolverHandle_t handle)
+{
+    Handle* h = H(handle);
+    if (h->laReady) return h->la.side ? &h->la : nullptr;
+    h->laReady = true;
+    LookAhead la;
+    bool ok = hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking) == hipSuccess;
+    hipEvent_t* evs[5] = {&la.fork, &la.panel[0], &la.panel[1], &la.bulk[0], &la.bulk[1]};
+    for (hipEvent_t* e : evs)
+        ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+    if (!ok)
+    {
+        (void)hipGetLastError();
+        for (hipEvent_t* e : evs)
+            if (*e) (void)hipEventDestroy(*e);
+        if (la.side) (void)hipStreamDestroy(la.side);
+        return nullptr;
+    }
+    h->la = la;
+    return &h->la;
+}
+}  // namespace ops
+
 OPS_API aclsolverStatus_t aclsolverDestroy(aclsolverHandle_t handle)
 {
     if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
     Handle* h = H(handle);
+    if (h->la.side != nullptr)
+    {
+        if (hipStreamSynchronize(h->la.side) != hipSuccess) return ACLSOLVER_STATUS_EXECUTION_FAILED;
+        hipEvent_t evs[5] = {h->la.fork, h->la.panel[0], h->la.panel[1], h->la.bulk[0], h->la.bulk[1]};
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(h->la.side);
+        h->la = LookAhead{};
+    }
     if (h->scratch != nullptr)
     {
         if (hipStreamSynchronize(h->stream) != hipSuccess) return ACLSOLVER_STATUS_EXECUTION_FAILED;
