@@ -112,9 +112,10 @@ extern "C"
     /* EXTENSION: refinement controls for the gesv family (tolerance on the backward error,
      * maximum outer refinement steps, GMRES restart length). */
     aclsolverStatus_t aclsolverSetRefinement(aclsolverHandle_t handle, double tolerance, int maxOuter, int restart);
-    /* EXTENSION: 1 (default) = SBgesv's trailing updates use split bf16 operands
-     * (x = hi + lo, products hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation);
-     * 0 = single bf16 operands. */
+    /* EXTENSION: operand split of SBgesv's trailing updates on bf16 MFMA (FP32 accumulation).
+     * 2 (default): x = h + m + l in bf16, six products (FP32-level products);
+     * 1: x = h + m, products h*h + h*m + m*h (~16 bits); 0: single bf16 operands.
+     * Returns INVALID_VALUE outside 0..2. */
     aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int split);
 
     /* ---- EXTENSION: uneven batched LU solve (getrf + getrs) in one persistent launch ----

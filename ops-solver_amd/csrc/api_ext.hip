@@ -105,16 +105,20 @@ OPS_API aclsolverStatus_t aclsolverSBgesv(aclsolverHandle_t handle, int n, int n
                                           float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
                                           size_t lwork_bytes, int* niter, int* dinfo)
 {
-    const Precision p = (handle != nullptr && H(handle)->bf16Split) ? Precision::BF16X3 : Precision::BF16;
+    const int split = handle != nullptr ? H(handle)->bf16Split : 2;
+    const Precision p = split >= 2 ? Precision::BF16X6 : (split == 1 ? Precision::BF16X3 : Precision::BF16);
     return Gesv(handle, p, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter, dinfo);
 }
 
-// EXTENSION: 1 (default) = SBgesv's trailing updates use the split-bf16 form
-// (hi*hi + hi*lo + lo*hi on bf16 MFMA, ~16-bit products); 0 = plain bf16 inputs.
+// EXTENSION: operand split of SBgesv's trailing updates on bf16 MFMA. 2 (default): three
+// bf16 parts per FP32 value, six products (~24-bit, FP32-level products); 1: two parts,
+// three products (~16 bits); 0: plain bf16 (8 bits). Fewer bits leave more work to the
+// refinement: at condition 1e6 only the three-part form keeps GMRES-IR short.
 OPS_API aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int split)
 {
     if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
-    H(handle)->bf16Split = split != 0;
+    if (split < 0 || split > 2) return ACLSOLVER_STATUS_INVALID_VALUE;
+    H(handle)->bf16Split = split;
     return ACLSOLVER_STATUS_SUCCESS;
 }
 OPS_API aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,

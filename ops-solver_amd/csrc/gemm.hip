@@ -47,13 +47,19 @@ __device__ inline float OpA(const float* A, int lda, int transA, int r, int k)
     return transA ? A[static_cast<size_t>(r) * lda + k] : A[static_cast<size_t>(k) * lda + r];
 }
 
+// FP32 update: 128 x 128 tile per workgroup, k chunks of 16 staged through LDS with the
+// next chunk prefetched into registers while the current one runs on the matrix cores
+// (one barrier per chunk). The product is formed as C^T (src0 = B, src1 = op(A)) so that
+// each accumulator register holds 32 consecutive rows of one column of C and the
+// read-modify-write of C is coalesced.
 template <int TRANSA>
 __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
                                                            const float* __restrict__ B, int ldb, float* __restrict__ C,
                                                            int ldc, int tilesM, int tilesN)
 {
-    __shared__ float As[16][TM + 4];
-    __shared__ float Bs[16][TN + 4];
+    constexpr int KC = 16;
+    __shared__ float As[2][KC][TM + 4];
+    __shared__ float Bs[2][KC][TN + 4];
     int tm, tn;
     TileOf(tilesM, tilesN, &tm, &tn);
     const int row0 = tm * TM, col0 = tn * TN;
@@ -67,76 +73,88 @@ __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(int M, int N, int K, 
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-    for (int k0 = 0; k0 < K; k0 += 16)
+    // Per chunk each thread moves 8 elements of op(A) and 8 of B.
+    float ra[8], rb[8];
+    auto gload = [&](int k0)
     {
-        if (!TRANSA)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
         {
-            const int r = tid & 127, kq = (tid >> 7) * 8;
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
+            int r, k;
+            if (!TRANSA)
             {
-                int k = kq + kk;
-                float v = 0.0f;
-                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(k0 + k) * lda + row0 + r];
-                As[k][r] = v;
+                r = tid & 127;
+                k = (tid >> 7) * 8 + q;
             }
+            else
+            {
+                r = tid >> 1;
+                k = (tid & 1) * 8 + q;
+            }
+            float v = 0.0f;
+            if (row0 + r < M && k0 + k < K)
+                v = TRANSA ? A[static_cast<size_t>(row0 + r) * lda + k0 + k] : A[static_cast<size_t>(k0 + k) * lda + row0 + r];
+            ra[q] = v;
+            const int j = tid >> 1, kb = (tid & 1) * 8 + q;
+            float u = 0.0f;
+            if (col0 + j < N && k0 + kb < K) u = B[static_cast<size_t>(col0 + j) * ldb + k0 + kb];
+            rb[q] = u;
         }
-        else
+    };
+    auto lstore = [&](int buf)
+    {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
         {
-            const int r = tid >> 1, kh = (tid & 1) * 8;
-#pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-            {
-                int k = kh + kk;
-                float v = 0.0f;
-                if (row0 + r < M && k0 + k < K) v = A[static_cast<size_t>(row0 + r) * lda + k0 + k];
-                As[k][r] = v;
-            }
+            if (!TRANSA)
+                As[buf][(tid >> 7) * 8 + q][tid & 127] = ra[q];
+            else
+                As[buf][(tid & 1) * 8 + q][tid >> 1] = ra[q];
+            Bs[buf][(tid & 1) * 8 + q][tid >> 1] = rb[q];
         }
-        {
-            const int j = tid >> 1, kh = (tid & 1) * 8;
+    };
+    const int chunks = (K + KC - 1) / KC;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int c = 0; c < chunks; ++c)
+    {
+        const int buf = c & 1;
+        if (c + 1 < chunks) gload((c + 1) * KC);
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk)
-            {
-                int k = kh + kk;
-                float v = 0.0f;
-                if (col0 + j < N && k0 + k < K) v = B[static_cast<size_t>(col0 + j) * ldb + k0 + k];
-                Bs[k][j] = v;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; kk += 2)
+        for (int kk = 0; kk < KC; kk += 2)
         {
             const int k = kk + (lane >> 5);
-            const float a0 = As[k][wm * 64 + (lane & 31)];
-            const float a1 = As[k][wm * 64 + 32 + (lane & 31)];
-            const float b0 = Bs[k][wn * 64 + (lane & 31)];
-            const float b1 = Bs[k][wn * 64 + 32 + (lane & 31)];
-            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+            const float a0 = As[buf][k][wm * 64 + (lane & 31)];
+            const float a1 = As[buf][k][wm * 64 + 32 + (lane & 31)];
+            const float b0 = Bs[buf][k][wn * 64 + (lane & 31)];
+            const float b1 = Bs[buf][k][wn * 64 + 32 + (lane & 31)];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(b0, a0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(b1, a0, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(b0, a1, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(b1, a1, acc[1][1], 0, 0, 0);
         }
+        if (c + 1 < chunks) lstore(buf ^ 1);
         __syncthreads();
     }
-    // 32x32 accumulator layout: register v of lane l holds row (v%4) + 8*(v/4) + 4*(l/32),
-    // column l%32 of the block.
+    // acc[bi][bj] register v of lane l: product row (v%4) + 8(v/4) + 4(l/32) -> column
+    // col0 + wn*64 + bj*32 + that of C; product column l%32 -> row row0 + wm*64 + bi*32 + l%32.
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
+    {
+        const int i = row0 + wm * 64 + bi * 32 + (lane & 31);
+        if (i >= M) continue;
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
         {
-            const int j = col0 + wn * 64 + bj * 32 + (lane & 31);
-            if (j >= N) continue;
-            float* cj = C + static_cast<size_t>(j) * ldc;
 #pragma unroll
             for (int v = 0; v < 16; ++v)
             {
-                const int i = row0 + wm * 64 + bi * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
-                if (i < M) cj[i] -= acc[bi][bj][v];
+                const int j = col0 + wn * 64 + bj * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+                if (j < N) C[static_cast<size_t>(j) * ldc + i] -= acc[bi][bj][v];
             }
         }
+    }
 }
 
 template <typename T>
@@ -480,7 +498,8 @@ hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, con
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     const int tilesM = (M + TM - 1) / TM, tilesN = (N + TN - 1) / TN;
     const int blocks = tilesM * tilesN;
-    if (prec == Precision::FP32)
+    // The three-part split is FP32-accurate; without packed operands it runs as FP32.
+    if (prec == Precision::FP32 || prec == Precision::BF16X6)
     {
         if (transA)
             hipLaunchKernelGGL(gemm_sub_f32_kernel<1>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb, C,

@@ -443,6 +443,12 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
                 ++totalIters;
                 if (std::fabs(g[j + 1]) <= 1e-4 * beta || !(hscal[S_HNEXT] > 0.0)) break;
             }
+            if (trace)
+            {
+                std::fprintf(stderr, "[gesv-ir]   gmres cycle: %d iterations, beta %.3e, |g|/beta %.3e, h(1,0) %.3e\n", k,
+                             beta, k > 0 ? std::fabs(g[k]) / beta : 1.0, restart > 0 ? H[0] : 0.0);
+                std::fflush(stderr);
+            }
             // back substitution H y = g
             std::vector<double> y(k, 0.0);
             for (int q = k - 1; q >= 0; --q)

@@ -33,7 +33,7 @@ struct Handle
     double irTolerance = 1e-6;
     int irMaxOuter = 30;
     int irRestart = 30;
-    int bf16Split = 1;        // SBgesv: split-bf16 (3-product) trailing updates
+    int bf16Split = 2;        // SBgesv trailing updates: 0 plain bf16, 1 two-part, 2 three-part split
     void* scratch = nullptr;  // handle-owned Device scratch (grown on demand)
     int* dflag = nullptr;     // Device flag word for validation scans
     int* hflag = nullptr;     // pinned host mirror of dflag
@@ -103,7 +103,8 @@ enum class Precision
     FP32 = 0,
     FP16 = 1,
     BF16 = 2,
-    BF16X3 = 3  // split bf16 (hi*hi + hi*lo + lo*hi) on bf16 MFMA
+    BF16X3 = 3,  // two-part split bf16 (h h' + h m' + m h'), ~16-bit products
+    BF16X6 = 4   // three-part split bf16 (six products), ~24-bit (FP32-level) products
 };
 // Workspace bytes needed by RealGetrf for an m x n matrix.
 size_t RealGetrfWorkspaceBytes(int m, int n);

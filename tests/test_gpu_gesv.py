@@ -66,3 +66,30 @@ def test_gesv_refined(torch, name, n, cond):
         assert niter.value >= 0, f"refinement did not converge ({niter.value})"
         assert berr <= 1e-6
     assert ferr <= max(cond * 1e-6, 1e-5) * 10
+
+
+@pytest.mark.parametrize("split", [0, 1, 2])
+def test_sbgesv_split_levels(torch, split):
+    """SBgesv at each bf16 operand split (0: plain, 1: two parts, 2: three parts) reaches
+    the refinement target on a moderately conditioned system; bad levels are rejected."""
+    n, cond = 1024, 1e3
+    a, at, b, x_true = _system(torch, n, cond, seed=77 + split)
+    with A.Handle() as h:
+        L = A.lib()
+        assert L.aclsolverSetBf16Split(h.h, 3) == A.INVALID_VALUE
+        assert L.aclsolverSetBf16Split(h.h, split) == A.SUCCESS
+        L.aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 30, 30)
+        lw = ctypes.c_size_t(0)
+        assert L.aclsolverSBgesv_bufferSize(h.h, n, 1, ctypes.byref(lw)) == A.SUCCESS
+        work = torch.empty(lw.value, dtype=torch.uint8, device="cuda")
+        x = torch.empty(n, device="cuda")
+        info = torch.full((1,), -5, dtype=torch.int32, device="cuda")
+        niter = ctypes.c_int(-100)
+        st = L.aclsolverSBgesv(h.h, n, 1, A.ptr(at), n, None, A.ptr(b), n, A.ptr(x), n, A.ptr(work), lw.value,
+                               ctypes.byref(niter), A.ptr(info))
+        torch.cuda.synchronize()
+    assert st == A.SUCCESS and info.item() == 0
+    assert niter.value >= 0, f"refinement did not converge ({niter.value})"
+    r = b.double() - a.double() @ x.double()
+    berr = (r.abs().max() / (a.double().abs().sum(1).max() * x.double().abs().max() + b.double().abs().max())).item()
+    assert berr <= 1e-6
