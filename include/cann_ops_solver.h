@@ -90,8 +90,12 @@ extern "C"
      * Argument convention of the reference's aclsolverSSgels (include/cann_ops_solver.h:
      * 815-862, the cuSOLVER IRS form). A is FP32 column-major and is NOT modified; the
      * factors live in dWorkspace. The trailing-matrix updates of the LU run on MFMA with
-     * FP16 (SH, per-panel power-of-two scaling) or BF16 (SB) inputs and FP32 accumulation;
-     * SS factors in plain FP32. Refinement: GMRES preconditioned by the LU factors, the
+     * FP16 (SH: each FP32 operand split into two FP16 parts after a power-of-two row /
+     * column scaling, three products) or BF16 (SB: split into three BF16 parts, six
+     * products, see aclsolverSetBf16Split) inputs and FP32 accumulation; SS factors in plain
+     * FP32, all with partial pivoting (see aclsolverSetButterfly for the unpivoted form).
+     * Refinement:
+     * GMRES preconditioned by the LU factors (applied by FP64 substitution), the
      * residual r = b - A x and the correction accumulation in FP64, until the backward
      * error ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= the handle's tolerance
      * (default 1e-6) or maxiter outer steps. niter (host) receives the total number of
@@ -117,6 +121,10 @@ extern "C"
      * 1: x = h + m, products h*h + h*m + m*h (~16 bits); 0: single bf16 operands.
      * Returns INVALID_VALUE outside 0..2. */
     aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int split);
+    /* EXTENSION: 1 = SH/SB factor U^T A V without pivoting after two levels of random
+     * butterfly transforms (n % 4 == 0; partial pivoting after a zero pivot or a refinement
+     * failure); 0 (default) = partial pivoting. */
+    aclsolverStatus_t aclsolverSetButterfly(aclsolverHandle_t handle, int on);
 
     /* ---- EXTENSION: uneven batched LU solve (getrf + getrs) in one persistent launch ----
      * nArray (HOST, batchSize entries) holds each order n_i; dAarray / dBarray are Device

@@ -53,6 +53,7 @@ def lib():
         "aclsolverSetStrictLimits": ([P, I], S),
         "aclsolverSetRefinement": ([P, ctypes.c_double, I, I], S),
         "aclsolverSetBf16Split": ([P, I], S),
+        "aclsolverSetButterfly": ([P, I], S),
         "aclsolverSgetrf_bufferSize": ([P, I, I, P, I, P], S),
         "aclsolverSgetrf": ([P, I, I, P, I, P, P, P], S),
         "aclsolverSgetri_bufferSize": ([P, I, P, I, P], S),

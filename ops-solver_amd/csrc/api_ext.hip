@@ -10,8 +10,8 @@ namespace ops
 {
 size_t GesvIrWorkspaceBytes(int n, int restart);
 hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
-                  void* work, Precision prec, double tol, int maxOuter, int restart, int* info, int* niterOut,
-                  hipStream_t stream, const LookAhead* la);
+                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, int* info,
+                  int* niterOut, hipStream_t stream, const LookAhead* la);
 }  // namespace ops
 
 using namespace ops;
@@ -76,7 +76,7 @@ aclsolverStatus_t Gesv(aclsolverHandle_t handle, Precision prec, int n, int nrhs
         piv = static_cast<int*>(extra);
     }
     if (GesvIR(n, nrhs, dA, ldda, piv, dB, lddb, dX, lddx, dWork, prec, h->irTolerance, h->irMaxOuter, h->irRestart,
-               dinfo, niter, s, HandleLookAhead(handle)) != hipSuccess)
+               h->butterfly != 0, dinfo, niter, s, HandleLookAhead(handle)) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }
@@ -98,7 +98,8 @@ OPS_API aclsolverStatus_t aclsolverSHgesv(aclsolverHandle_t handle, int n, int n
                                           float* dB, int lddb, float* dX, int lddx, void* dWorkspace,
                                           size_t lwork_bytes, int* niter, int* dinfo)
 {
-    return Gesv(handle, Precision::FP16, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
+    // FP16 matrix cores on the two-part split with power-of-two row/column scaling.
+    return Gesv(handle, Precision::FP16X3, n, nrhs, dA, ldda, dipiv, dB, lddb, dX, lddx, dWorkspace, lwork_bytes, niter,
                 dinfo);
 }
 OPS_API aclsolverStatus_t aclsolverSBgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,
@@ -119,6 +120,15 @@ OPS_API aclsolverStatus_t aclsolverSetBf16Split(aclsolverHandle_t handle, int sp
     if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
     if (split < 0 || split > 2) return ACLSOLVER_STATUS_INVALID_VALUE;
     H(handle)->bf16Split = split;
+    return ACLSOLVER_STATUS_SUCCESS;
+}
+// EXTENSION: 1 = SH/SB factor U^T A V without pivoting after two levels of random
+// butterfly transforms (n % 4 == 0), falling back to partial pivoting after a zero pivot or
+// a refinement failure; 0 (default) = partial pivoting.
+OPS_API aclsolverStatus_t aclsolverSetButterfly(aclsolverHandle_t handle, int on)
+{
+    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    H(handle)->butterfly = on != 0;
     return ACLSOLVER_STATUS_SUCCESS;
 }
 OPS_API aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,

@@ -498,8 +498,8 @@ hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, con
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     const int tilesM = (M + TM - 1) / TM, tilesN = (N + TN - 1) / TN;
     const int blocks = tilesM * tilesN;
-    // The three-part split is FP32-accurate; without packed operands it runs as FP32.
-    if (prec == Precision::FP32 || prec == Precision::BF16X6)
+    // The split forms are (near) FP32-accurate; without packed operands they run as FP32.
+    if (prec == Precision::FP32 || prec == Precision::BF16X6 || prec == Precision::FP16X3)
     {
         if (transA)
             hipLaunchKernelGGL(gemm_sub_f32_kernel<1>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb, C,

@@ -14,6 +14,9 @@
 // (src0 = U fragment, src1 = L fragment) so that each accumulator register maps to 32
 // consecutive rows of one column of C: the read-modify-write of C is coalesced.
 //
+// FP16 form: the two-part split in FP16 (about 22 bits per product), every row of the L
+// panel and column of the U block row scaled by a power of two first (FP16's range is
+// narrow), the factors undone exactly in the epilogue.
 // Split forms (FP32 accumulation): two parts x = h + m (hi = bf16(x), m = bf16(x - h)),
 // x*y ~ h h' + h m' + m h' (about 16 significant bits per product); three parts
 // x = h + m + l, x*y ~ h h' + h m' + m h' + h l' + m m' + l h' (about 24 bits, the
@@ -30,6 +33,46 @@ namespace
 {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// Element type of the packed operands and the matching MFMA.
+template <typename ET>
+struct Mfma;
+template <>
+struct Mfma<__bf16>
+{
+    typedef bf16x8 vec;
+    static __device__ inline f32x16 run(vec a, vec b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma<_Float16>
+{
+    typedef f16x8 vec;
+    static __device__ inline f32x16 run(vec a, vec b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0); }
+};
+
+// Power of two that brings a row's largest magnitude into [2^14, 2^15) (FP16 keeps 2^15 <
+// 65504); clamped so the factor and its inverse stay finite normal floats.
+__device__ inline float RowScale(float maxAbs)
+{
+    if (!(maxAbs > 0.0f) || !(maxAbs <= 3.0e38f)) return 1.0f;
+    int e = 14 - ilogbf(maxAbs);
+    e = e > 100 ? 100 : (e < -100 ? -100 : e);
+    return ldexpf(1.0f, e);
+}
+
+// scale[r] = RowScale(max_k |X(r, k)|) for X(r, k) = src[r * rs + k * ks]; zero-padded
+// rows past `rows` get 1.
+__global__ __launch_bounds__(256) void row_scale_kernel(int rows, int padded, int K, const float* __restrict__ src,
+                                                        int64_t rs, int64_t ks, float* __restrict__ scale)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= padded) return;
+    float m = 0.0f;
+    if (r < rows)
+        for (int k = 0; k < K; ++k) m = fmaxf(m, fabsf(src[r * rs + k * ks]));
+    scale[r] = RowScale(m);
+}
 
 constexpr int kTile = 128;  // C tile per workgroup (rows and columns)
 constexpr int kFragRows = 32;
@@ -43,30 +86,33 @@ constexpr int StageK()
 
 inline int RoundUp(int v, int m) { return (v + m - 1) / m * m; }
 
-// Packs the `rows x K` operand X(r, k) = src[r * rs + k * ks] into fragment order:
-// fragment f = rb * KB + kb holds, for lane l, X(rb*32 + l%32, kb*16 + (l/32)*8 + i),
-// i < 8, as PARTS split vectors (part 0 = bf16(x), part p = bf16 of what parts < p leave).
-// Out-of-range elements are zero.
-template <int PARTS>
+// Packs the `rows x K` operand X(r, k) = src[r * rs + k * ks] (times scale[r] when given)
+// into fragment order: fragment f = rb * KB + kb holds, for lane l,
+// X(rb*32 + l%32, kb*16 + (l/32)*8 + i), i < 8, as PARTS split vectors (part 0 = ET(x),
+// part p = ET of what parts < p leave). Out-of-range elements are zero.
+template <typename ET, int PARTS>
 __global__ __launch_bounds__(256) void pack_frag_kernel(int rows, int K, const float* __restrict__ src, int64_t rs,
-                                                        int64_t ks, int KB, int nfrag, bf16x8* __restrict__ out)
+                                                        int64_t ks, int KB, int nfrag, const float* __restrict__ scale,
+                                                        typename Mfma<ET>::vec* __restrict__ out)
 {
+    typedef typename Mfma<ET>::vec VT;
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int f = t >> 6, l = t & 63;
     if (f >= nfrag) return;
     const int rb = f / KB, kb = f % KB;
     const int r = rb * kFragRows + (l & 31);
     const int k0 = kb * kFragK + (l >> 5) * 8;
-    bf16x8 part[PARTS];
+    const float sc = (scale != nullptr && r < rows) ? scale[r] : 1.0f;
+    VT part[PARTS];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
     {
         const int k = k0 + i;
-        float v = (r < rows && k < K) ? src[r * rs + k * ks] : 0.0f;
+        float v = (r < rows && k < K) ? src[r * rs + k * ks] * sc : 0.0f;
 #pragma unroll
         for (int p = 0; p < PARTS; ++p)
         {
-            const __bf16 h = static_cast<__bf16>(v);
+            const ET h = static_cast<ET>(v);
             part[p][i] = h;
             v -= static_cast<float>(h);
         }
@@ -92,15 +138,18 @@ __device__ inline void TileOfXcd(int tilesM, int tilesN, int* tm, int* tn)
 
 // One workgroup (4 waves) per 128 x 128 tile of C; wave (wm, wn) owns the 64 x 64
 // quadrant rows [wm*64, +64), columns [wn*64, +64) as 2 x 2 accumulators of 32 x 32.
-template <int PARTS>
-__global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, const bf16x8* __restrict__ Ap,
-                                                          const bf16x8* __restrict__ Bp, float* __restrict__ C, int ldc,
-                                                          int tilesM, int tilesN)
+template <typename ET, int PARTS>
+__global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, const typename Mfma<ET>::vec* __restrict__ Ap,
+                                                          const typename Mfma<ET>::vec* __restrict__ Bp,
+                                                          float* __restrict__ C, int ldc, int tilesM, int tilesN,
+                                                          const float* __restrict__ rowScale,
+                                                          const float* __restrict__ colScale)
 {
+    typedef typename Mfma<ET>::vec VT;
     constexpr int SK = StageK<PARTS>();
     // [stage][block 0..3][kk][part][lane]
-    __shared__ bf16x8 sA[2][4][SK][PARTS][64];
-    __shared__ bf16x8 sB[2][4][SK][PARTS][64];
+    __shared__ VT sA[2][4][SK][PARTS][64];
+    __shared__ VT sB[2][4][SK][PARTS][64];
     int tm, tn;
     TileOfXcd(tilesM, tilesN, &tm, &tn);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -119,7 +168,7 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
     // each); thread t moves vector t + 256 q of each.
     constexpr int VEC = 4 * SK * PARTS * 64;  // vectors per operand per stage
     constexpr int PER = VEC / 256;
-    bf16x8 ra[PER], rbv[PER];
+    VT ra[PER], rbv[PER];
     auto gload = [&](int stage)
     {
 #pragma unroll
@@ -157,7 +206,7 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
 #pragma unroll
         for (int kk = 0; kk < SK; ++kk)
         {
-            bf16x8 a[PARTS][2], bb[PARTS][2];
+            VT a[PARTS][2], bb[PARTS][2];
 #pragma unroll
             for (int p = 0; p < PARTS; ++p)
 #pragma unroll
@@ -173,7 +222,7 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
                 {
                     // C^T tile: rows of the product are columns of C (src0 = U fragment).
                     // Smallest terms first: l h', m m', h l', m h', h m', h h'.
-#define MF(X, Y) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bb[X][j], a[Y][i], acc[i][j], 0, 0, 0)
+#define MF(X, Y) acc[i][j] = Mfma<ET>::run(bb[X][j], a[Y][i], acc[i][j])
                     if constexpr (PARTS == 3)
                     {
                         MF(0, 2);
@@ -195,11 +244,13 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
 
     // Accumulator register v of lane l: product row (v%4) + 8(v/4) + 4(l/32) -> column of
     // C, product column l%32 -> row of C.
+    // Scaled operands: undo the power-of-two row (A) and column (B) factors exactly.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
     {
         const int row = (rb0 + wm * 2 + i) * kFragRows + (lane & 31);
         if (row >= M) continue;
+        const float ra_inv = rowScale != nullptr ? 1.0f / rowScale[row] : 1.0f;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
         {
@@ -211,7 +262,8 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
                 if (col < N)
                 {
                     float* p = C + static_cast<size_t>(col) * ldc + row;
-                    *p -= acc[i][j][v];
+                    const float prod = colScale != nullptr ? (acc[i][j][v] * ra_inv) * (1.0f / colScale[col]) : acc[i][j][v];
+                    *p -= prod;
                 }
             }
         }
@@ -243,40 +295,56 @@ size_t GemmPackedWorkspaceBytes(int M, int N, int K)
 {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     const PackLayout p = Layout(M, N, K, 3);
-    return (p.aVecs + p.bVecs) * sizeof(bf16x8) + 256;
+    return (p.aVecs + p.bVecs) * sizeof(bf16x8) + (static_cast<size_t>(p.rowBlocks) + p.colBlocks) * kFragRows * 4 +
+           512;
 }
 
 hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
                          Precision prec, void* work, hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-    if (prec != Precision::BF16 && prec != Precision::BF16X3 && prec != Precision::BF16X6)
+    if (prec != Precision::BF16 && prec != Precision::BF16X3 && prec != Precision::BF16X6 &&
+        prec != Precision::FP16X3)
         return GemmSub(M, N, K, A, lda, 0, B, ldb, C, ldc, prec, stream);
-    const int parts = prec == Precision::BF16X6 ? 3 : (prec == Precision::BF16X3 ? 2 : 1);
+    const int parts = prec == Precision::BF16X6 ? 3 : ((prec == Precision::BF16X3 || prec == Precision::FP16X3) ? 2 : 1);
     const PackLayout p = Layout(M, N, K, parts);
-    bf16x8* ap = static_cast<bf16x8*>(work);
-    bf16x8* bp = ap + p.aVecs;
+    char* base = static_cast<char*>(work);
+    void* ap = base;
+    void* bp = base + p.aVecs * sizeof(bf16x8);
+    float* sa = reinterpret_cast<float*>(base + (p.aVecs + p.bVecs) * sizeof(bf16x8));
+    float* sb = sa + static_cast<size_t>(p.rowBlocks) * kFragRows;
     const int fa = p.rowBlocks * p.KB, fb = p.colBlocks * p.KB;
     const int tilesM = p.rowBlocks / 4, tilesN = p.colBlocks / 4;
     const dim3 grid(tilesM * tilesN);
+    const int64_t one = 1, la = lda, lb = ldb;
     // A(r, k) = A[k * lda + r]; B^T(c, k) = B[c * ldb + k]
-#define PACK_AND_RUN(P)                                                                                           \
-    hipLaunchKernelGGL(pack_frag_kernel<P>, dim3((fa * 64 + 255) / 256), dim3(256), 0, stream, M, K, A, int64_t{1}, \
-                       static_cast<int64_t>(lda), p.KB, fa, ap);                                                 \
-    hipLaunchKernelGGL(pack_frag_kernel<P>, dim3((fb * 64 + 255) / 256), dim3(256), 0, stream, N, K, B,           \
-                       static_cast<int64_t>(ldb), int64_t{1}, p.KB, fb, bp);                                     \
-    hipLaunchKernelGGL(gemm_packed_kernel<P>, grid, dim3(256), 0, stream, M, N, p.KB, ap, bp, C, ldc, tilesM, tilesN)
-    if (parts == 3)
+#define PACK_AND_RUN(ET, P, SA, SB)                                                                                 \
+    hipLaunchKernelGGL((pack_frag_kernel<ET, P>), dim3((fa * 64 + 255) / 256), dim3(256), 0, stream, M, K, A, one, la, \
+                       p.KB, fa, SA, static_cast<Mfma<ET>::vec*>(ap));                                            \
+    hipLaunchKernelGGL((pack_frag_kernel<ET, P>), dim3((fb * 64 + 255) / 256), dim3(256), 0, stream, N, K, B, lb, one, \
+                       p.KB, fb, SB, static_cast<Mfma<ET>::vec*>(bp));                                            \
+    hipLaunchKernelGGL((gemm_packed_kernel<ET, P>), grid, dim3(256), 0, stream, M, N, p.KB,                       \
+                       static_cast<const Mfma<ET>::vec*>(ap), static_cast<const Mfma<ET>::vec*>(bp), C, ldc, tilesM, \
+                       tilesN, SA, SB)
+    if (prec == Precision::FP16X3)
     {
-        PACK_AND_RUN(3);
+        // FP16 has 5 exponent bits: scale every row of A and column of B by a power of two.
+        const int pa = p.rowBlocks * kFragRows, pb = p.colBlocks * kFragRows;
+        hipLaunchKernelGGL(row_scale_kernel, dim3((pa + 255) / 256), dim3(256), 0, stream, M, pa, K, A, one, la, sa);
+        hipLaunchKernelGGL(row_scale_kernel, dim3((pb + 255) / 256), dim3(256), 0, stream, N, pb, K, B, lb, one, sb);
+        PACK_AND_RUN(_Float16, 2, sa, sb);
+    }
+    else if (parts == 3)
+    {
+        PACK_AND_RUN(__bf16, 3, nullptr, nullptr);
     }
     else if (parts == 2)
     {
-        PACK_AND_RUN(2);
+        PACK_AND_RUN(__bf16, 2, nullptr, nullptr);
     }
     else
     {
-        PACK_AND_RUN(1);
+        PACK_AND_RUN(__bf16, 1, nullptr, nullptr);
     }
 #undef PACK_AND_RUN
     return hipGetLastError();

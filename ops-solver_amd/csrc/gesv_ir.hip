@@ -253,8 +253,8 @@ size_t GesvIrWorkspaceBytes(int n, int restart) { return IrLayout(nullptr, n, re
 // Returns hipSuccess; *niterOut = total GMRES iterations (negative when refinement did not
 // reach tol within maxOuter outer steps).
 hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
-                  void* work, Precision prec, double tol, int maxOuter, int restart, int* info, int* niterOut,
-                  hipStream_t stream, const LookAhead* la)
+                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, int* info,
+                  int* niterOut, hipStream_t stream, const LookAhead* la)
 {
     IrWork w = IrLayout(work, n, restart);
     // OPS_SOLVER_TRACE=1 prints one line per refinement step to stderr (the loop already
@@ -263,9 +263,10 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
     const int chunks = (n + CB - 1) / CB;
-    // The mixed precisions factor U^T A V without pivoting (random butterflies, n % 4 == 0);
-    // FP32, other orders and the fallback after a refinement failure use partial pivoting.
-    bool useRbt = prec != Precision::FP32 && n % 4 == 0 && n >= 8;
+    // Partial pivoting by default. With `butterfly` the mixed precisions factor U^T A V
+    // without pivoting (random butterflies, n % 4 == 0) and fall back to pivoting after a
+    // zero pivot or a refinement failure.
+    bool useRbt = butterfly && prec != Precision::FP32 && n % 4 == 0 && n >= 8;
     hipError_t e = hipSuccess;
     auto factor = [&]() -> hipError_t
     {

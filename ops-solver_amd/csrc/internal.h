@@ -34,6 +34,7 @@ struct Handle
     int irMaxOuter = 30;
     int irRestart = 30;
     int bf16Split = 2;        // SBgesv trailing updates: 0 plain bf16, 1 two-part, 2 three-part split
+    int butterfly = 0;        // SH/SB: random butterflies + no pivoting instead of partial pivoting
     void* scratch = nullptr;  // handle-owned Device scratch (grown on demand)
     int* dflag = nullptr;     // Device flag word for validation scans
     int* hflag = nullptr;     // pinned host mirror of dflag
@@ -104,7 +105,8 @@ enum class Precision
     FP16 = 1,
     BF16 = 2,
     BF16X3 = 3,  // two-part split bf16 (h h' + h m' + m h'), ~16-bit products
-    BF16X6 = 4   // three-part split bf16 (six products), ~24-bit (FP32-level) products
+    BF16X6 = 4,  // three-part split bf16 (six products), ~24-bit (FP32-level) products
+    FP16X3 = 5   // two-part split fp16 with power-of-two row/column scaling, ~22-bit products
 };
 // Workspace bytes needed by RealGetrf for an m x n matrix.
 size_t RealGetrfWorkspaceBytes(int m, int n);

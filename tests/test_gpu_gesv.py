@@ -1,5 +1,5 @@
-"""GPU checks of the mixed-precision gesv family (SH: FP16, SB: BF16 MFMA trailing updates
-+ GMRES-IR with FP64 residuals; SS: plain FP32) through the C ABI.
+"""GPU checks of the mixed-precision gesv family (SH: split-FP16, SB: split-BF16 MFMA trailing
+updates + GMRES-IR with FP64 residuals; SS: plain FP32) through the C ABI.
 
 Bar: backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= 1e-6 (the
 refinement target), forward error against the known FP64 solution within cond * 1e-6,
@@ -93,3 +93,31 @@ def test_sbgesv_split_levels(torch, split):
     r = b.double() - a.double() @ x.double()
     berr = (r.abs().max() / (a.double().abs().sum(1).max() * x.double().abs().max() + b.double().abs().max())).item()
     assert berr <= 1e-6
+
+
+def test_sbgesv_butterfly_option(torch):
+    """The unpivoted random-butterfly form (opt-in) still reaches the refinement target on a
+    moderately conditioned system, and the ipiv it reports is the identity or a valid
+    pivot sequence from the pivoted fallback."""
+    n, cond = 1024, 1e3
+    a, at, b, x_true = _system(torch, n, cond, seed=91)
+    with A.Handle() as h:
+        L = A.lib()
+        assert L.aclsolverSetButterfly(h.h, 1) == A.SUCCESS
+        L.aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 30, 30)
+        lw = ctypes.c_size_t(0)
+        assert L.aclsolverSBgesv_bufferSize(h.h, n, 1, ctypes.byref(lw)) == A.SUCCESS
+        work = torch.empty(lw.value, dtype=torch.uint8, device="cuda")
+        x = torch.empty(n, device="cuda")
+        ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+        info = torch.full((1,), -5, dtype=torch.int32, device="cuda")
+        niter = ctypes.c_int(-100)
+        st = L.aclsolverSBgesv(h.h, n, 1, A.ptr(at), n, A.ptr(ipiv), A.ptr(b), n, A.ptr(x), n, A.ptr(work),
+                               lw.value, ctypes.byref(niter), A.ptr(info))
+        torch.cuda.synchronize()
+    assert st == A.SUCCESS and info.item() == 0 and niter.value >= 0
+    r = b.double() - a.double() @ x.double()
+    berr = (r.abs().max() / (a.double().abs().sum(1).max() * x.double().abs().max() + b.double().abs().max())).item()
+    assert berr <= 1e-6
+    piv = ipiv.cpu().numpy()
+    assert ((piv >= np.arange(1, n + 1)) & (piv <= n)).all()
