@@ -1,19 +1,20 @@
 // Uneven batched LU solve (getrf + getrs per matrix) in one persistent launch (EXTENSION;
 // the reference has no batched getrf/getrs).
 //
-// Plan: matrices are binned by order into 64 / 128 / 256 row classes. A work item is a
-// group of same-class matrices that fills one 1024-thread workgroup (16 x n<=64 with one
-// thread per row, 4 x n<=128 with two, 1 x n<=256 with four), so every thread keeps 64
-// matrix values in VGPRs; items are queued largest class first. A persistent grid pulls items from
-// a device counter until the queue is empty, so every workgroup reaches the exit.
+// Plan: matrices are binned by order into 64 / 128 / 256 row classes; a work item fills one
+// 512-thread workgroup (8 matrices of n <= 64 with one wave each, 4 of n <= 128 with two
+// waves each, or 1 of n <= 256 with two threads per row), and every thread keeps at most
+// 128 matrix values in VGPRs. Items are queued largest class first, larger orders first
+// (longest-processing-time order). A persistent grid pulls items from a device counter
+// until the queue is empty, so every workgroup reaches the exit. The autotuner picks the
+// grid size at plan time.
 //
-// Per matrix: every row lives in one thread's registers (row-per-thread, position-tracked
-// partial pivoting exactly as the register leaf of getrf.hip: largest |a|, lowest position
-// on ties, reciprocal scaling, fused rank-1 updates), the right-hand sides ride along
-// with their rows, then a back substitution on the final positions. The column index k is
-// a runtime value but wave-uniform, so each static-column operation sits behind a scalar
-// branch instead of dynamic register indexing. Outputs: packed LU (rows stored at their
-// final positions), 1-based ipiv, X, info (first zero pivot).
+// Per matrix: rows live in registers (position-tracked partial pivoting exactly as the
+// register leaf of getrf.hip: largest |a|, lowest position on ties, reciprocal scaling,
+// fused rank-1 updates); per column a 64-bit key argmax (DPP in the wave, one slot per wave
+// through LDS) and two barriers (one wave: wave-level only); the right-hand sides ride
+// along with their rows, then a back substitution in 32-column blocks. Outputs: packed LU
+// (rows stored at their final positions), 1-based ipiv, X, info (first zero pivot).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,215 +31,333 @@ namespace ops
 namespace
 {
 constexpr int kNrhsMax = 4;  // right-hand sides per matrix carried in registers
-constexpr int kWG = 1024;
+constexpr int kWG = 512;
+constexpr int kBS = 32;      // back-substitution block
 
 struct Item
 {
-    int bin;     // 0: n<=64, 1: n<=128, 2: n<=256
+    int bin;     // 0: n<=64 (8 per item), 1: n<=128 (4 per item), 2: n<=256 (1 per item)
     int first;   // index into the plan's matrix order
     int count;   // matrices in this item
 };
 
-// One matrix processed by G = NMAX * TPR threads. Thread t owns row r = t % NMAX and, when
-// TPR = 2, the half h = t / NMAX of its columns (interleaved: column j = TPR * jj + h), so
-// each thread keeps NMAX / TPR values in registers. The thread holding column k of a row
-// computes that row's multiplier and shares it through LDS with the row's other thread.
-template <int NMAX, int TPR>
-__device__ void solve_group(int n, int nrhs, float* __restrict__ A, float* __restrict__ B, int* __restrict__ ipiv,
-                            int* __restrict__ info, int t, bool live, float* s_prow, float* s_rhs, float* s_best,
-                            int* s_bpos, float* s_l)
+__device__ inline unsigned long long KeyOf(float v, int pos)
 {
-    constexpr int G = NMAX * TPR;
+    const float a = fabsf(v);
+    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;  // NaN never wins
+    return (1ull << 63) | (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~pos);
+}
+
+template <int CTRL>
+__device__ inline unsigned long long Dpp64(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xffffffffull), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, 0xF, 0xF, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+__device__ inline unsigned long long Lane64(unsigned long long v, int lane)
+{
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), lane);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+__device__ inline unsigned long long WaveMax64(unsigned long long v)
+{
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    v = mx(v, Dpp64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, Dpp64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, Dpp64<0x124>(v));  // row_ror:4
+    v = mx(v, Dpp64<0x128>(v));  // row_ror:8
+    return mx(mx(Lane64(v, 0), Lane64(v, 16)), mx(Lane64(v, 32), Lane64(v, 48)));
+}
+
+// LDS of one group (one matrix).
+template <int NMAX, int TPR>
+struct GroupLds
+{
+    static constexpr int NWG = NMAX * TPR / 64;  // waves in the group
+    static constexpr int C = NMAX / TPR;         // columns per thread
+    unsigned long long key[NWG];
+    float val[NWG];
+    float prow[TPR][C];  // pivot row, de-interleaved by column owner
+    float rhs[kNrhsMax];
+    float l[TPR > 1 ? NMAX : 1];
+    float U[kBS][kBS + 1];
+    float y[kBS][kNrhsMax];
+    float part[TPR > 1 ? NMAX : 1][kNrhsMax];
+};
+
+// LU with partial pivoting of one n x n matrix (n <= NMAX) plus the solve of nrhs right-hand
+// sides, by G = NMAX * TPR threads. Thread t owns row r = t % NMAX and the columns
+// j = TPR * jj + h, h = t / NMAX (C = NMAX / TPR of them) in registers. Pivoting is
+// implicit: rows keep their registers and track the position they occupy; the pivot of
+// column k is the candidate with the largest |a| and, on ties, the lowest position (the
+// LAPACK first-row rule), scaling is by the reciprocal when |d| >= FLT_MIN (division
+// otherwise), updates are fused multiply-adds -- as in the register leaf of the dense
+// factorization. The right-hand sides ride along with their rows (forward substitution
+// during elimination); back substitution runs in blocks of 32 columns.
+template <int NMAX, int TPR, bool WAVE>
+__device__ __attribute__((always_inline)) void solve_group(int n, int nLoop, int nrhs, float* __restrict__ A,
+                                                         float* __restrict__ B, int* __restrict__ ipiv,
+                                                         int* __restrict__ info, int t, bool live,
+                                                         GroupLds<NMAX, TPR>& L)
+{
+    // nLoop: the largest order among the groups that synchronize together (workgroup
+    // barriers), so every group runs the same number of barrier steps.
     constexpr int C = NMAX / TPR;
-    constexpr bool WAVE = G == 64;
-    constexpr int NWG = G / 64;
+    constexpr int NWG = NMAX * TPR / 64;
+    constexpr int CH = 8;  // columns per update chunk (per thread)
     auto sync = [&]()
     {
         if (WAVE)
         {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         else
             __syncthreads();
     };
-    // NMAX is a multiple of 64, so the column half h is uniform across a wave: keeping it
-    // in a scalar register turns every column test below into a scalar branch.
-    const int r = t % NMAX, h = __builtin_amdgcn_readfirstlane(t / NMAX);
+    const int r = t % NMAX;
+    const int h = __builtin_amdgcn_readfirstlane(t / NMAX);  // uniform per wave (NMAX % 64 == 0)
+    const int wig = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lane = t & 63;
     const bool rowLive = live && r < n;
-    const bool rhsOwner = h == 0;
     float a[C];
-    float b[kNrhsMax];
-#pragma unroll
-    for (int jj = 0; jj < C; ++jj)
     {
-        const int j = TPR * jj + h;
-        a[jj] = (rowLive && j < n) ? A[static_cast<size_t>(j) * n + r] : 0.0f;
-    }
+        // Loads in chunks: each chunk's addresses die before the next chunk issues.
+        const float* src = A + r + static_cast<size_t>(h) * n;
+        const size_t stride = static_cast<size_t>(TPR) * n;
 #pragma unroll
-    for (int q = 0; q < kNrhsMax; ++q) b[q] = (rowLive && rhsOwner && q < nrhs) ? B[static_cast<size_t>(q) * n + r] : 0.0f;
+        for (int c0 = 0; c0 < C; c0 += 16)
+        {
+#pragma unroll
+            for (int jj = c0; jj < c0 + 16; ++jj)
+            {
+                const int j = TPR * jj + h;
+                a[jj] = (rowLive && j < n) ? src[jj * stride] : 0.0f;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    float b[kNrhsMax];  // every thread of a row carries the right-hand sides
+#pragma unroll
+    for (int q = 0; q < kNrhsMax; ++q) b[q] = (rowLive && q < nrhs) ? B[static_cast<size_t>(q) * n + r] : 0.0f;
     int pos = rowLive ? r : INT_MAX;
-    float cur = a[0];  // this thread's value in column k (valid when it owns column k)
+    float cur = a[0];  // value in column k when this thread owns column k
     int zero = 0;
-    const int lane = threadIdx.x & 63;
-    const int wig = t >> 6;
     const int steps = live ? n : 0;
+
     for (int k = 0; k < NMAX; ++k)
     {
-        if (WAVE && k >= steps) break;  // wave-private groups stop independently
+        if (WAVE && k >= steps) break;   // a wave-private group stops on its own
+        if (!WAVE && k >= nLoop) break;  // uniform across the workgroup
         const bool active = k < steps;
         const bool ownK = (k % TPR) == h;
-        float best = (active && ownK && pos >= k && pos != INT_MAX) ? fabsf(cur) : -1.0f;
-        int bp = (best >= 0.0f) ? pos : INT_MAX;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
+        // (1) candidates of column k: the wave's best key, then one slot per wave
+        const unsigned long long myKey =
+            (active && ownK && pos >= k && pos != INT_MAX) ? KeyOf(cur, pos) : 0ull;
+        const unsigned long long wmax = WaveMax64(myKey);
+        if (myKey != 0ull && myKey == wmax)
         {
-            const float ob = __shfl_xor(best, off);
-            const int op = __shfl_xor(bp, off);
-            if (ob > best || (ob == best && op < bp))
+            L.key[wig] = wmax;
+            L.val[wig] = cur;
+        }
+        if (lane == 0 && wmax == 0ull) L.key[wig] = 0ull;
+        sync();
+        unsigned long long top = L.key[0];
+        int tw = 0;
+#pragma unroll
+        for (int w = 1; w < NWG; ++w)
+        {
+            const unsigned long long kw = L.key[w];
+            if (kw > top)
             {
-                best = ob;
-                bp = op;
+                top = kw;
+                tw = w;
             }
         }
-        if (!WAVE)
-        {
-            if (lane == 0)
-            {
-                s_best[wig] = best;
-                s_bpos[wig] = bp;
-            }
-            sync();
-            best = s_best[0];
-            bp = s_bpos[0];
-#pragma unroll
-            for (int w = 1; w < NWG; ++w)
-            {
-                const float ob = s_best[w];
-                const int op = s_bpos[w];
-                if (ob > best || (ob == best && op < bp))
-                {
-                    best = ob;
-                    bp = op;
-                }
-            }
-        }
-        const int bpos = (bp == INT_MAX) ? k : bp;
+        const int bpos = top == 0ull ? k : static_cast<int>(~static_cast<unsigned>(top & 0xffffffffull));
+        const float d = top == 0ull ? 0.0f : L.val[tw];
+        // (2) publish the pivot row and this row's multiplier
+        float l = 0.0f;
         if (active)
         {
             if (pos == bpos)
             {
 #pragma unroll
-                for (int jj = 0; jj < C; ++jj) s_prow[TPR * jj + h] = a[jj];
-                if (rhsOwner)
-                {
+                for (int jj = 0; jj < C; ++jj) L.prow[h][jj] = a[jj];
+                if (h == 0)
 #pragma unroll
-                    for (int q = 0; q < kNrhsMax; ++q) s_rhs[q] = b[q];
-                }
+                    for (int q = 0; q < kNrhsMax; ++q) L.rhs[q] = b[q];
             }
-            if (t == 0) ipiv[k] = bpos + 1;
             if (pos == bpos)
                 pos = k;
             else if (pos == k)
                 pos = bpos;
-        }
-        sync();
-        float l = 0.0f;
-        if (active)
-        {
-            const float d = s_prow[k];
+            if (t == 0) ipiv[k] = bpos + 1;
             if (d == 0.0f && zero == 0) zero = k + 1;
-            const bool below = pos > k && pos != INT_MAX && d != 0.0f;
-            const float q = fabsf(d) >= FLT_MIN ? cur * (1.0f / d) : cur / d;
             if (ownK)
             {
+                const bool below = pos > k && pos != INT_MAX && d != 0.0f;
+                const float q = fabsf(d) >= FLT_MIN ? cur * (1.0f / d) : cur / d;
                 l = below ? q : 0.0f;
-                if (TPR > 1) s_l[r] = l;
+                if (TPR > 1) L.l[r] = l;
+            }
+        }
+        sync();
+        // (3) rank-1 update of the columns right of k (whole chunks left of k are skipped)
+        if (active)
+        {
+            if (TPR > 1 && !ownK) l = L.l[r];
+            float next = 0.0f;
 #pragma unroll
-                for (int jj = 0; jj < C; ++jj)
-                    if (TPR * jj + h == k && below) a[jj] = q;
+            for (int c0 = 0; c0 < C; c0 += CH)
+            {
+                if (TPR * (c0 + CH) <= k) continue;  // uniform: every column of the chunk is < k
+#pragma unroll
+                for (int jj = c0; jj < c0 + CH; ++jj)
+                {
+                    const int j = TPR * jj + h;
+                    const float u = L.prow[h][jj];
+                    if (j > k) a[jj] = fmaf(-l, u, a[jj]);
+                    if (j == k && ownK && l != 0.0f) a[jj] = l;  // store the multiplier
+                    if (j == k + 1) next = a[jj];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // keep each chunk's pivot-row loads local
+            }
+#pragma unroll
+            for (int q = 0; q < kNrhsMax; ++q) b[q] = fmaf(-l, L.rhs[q], b[q]);
+            cur = next;
+        }
+    }
+
+    // Back substitution U x = y over final positions, blocks of kBS columns from the end of
+    // this matrix; the number of block steps is uniform across the synchronizing groups.
+    const int nb = live ? n : 0;
+    const int blocks = ((WAVE ? nb : nLoop) + kBS - 1) / kBS;
+    for (int bi = 0; bi < blocks; ++bi)
+    {
+        const int c1 = nb - bi * kBS;  // <= 0: this matrix has no block left (barriers only)
+        const int c0 = c1 - kBS > 0 ? c1 - kBS : 0;
+        const bool inBlk = rowLive && pos >= c0 && pos < c1;
+        // (a) diagonal block rows and their right-hand sides to LDS
+        if (inBlk)
+        {
+#pragma unroll
+            for (int cc = 0; cc < C; cc += CH)
+            {
+                if (TPR * (cc + CH) <= c0 || TPR * cc >= c1) continue;  // uniform
+#pragma unroll
+                for (int jj = cc; jj < cc + CH; ++jj)
+                {
+                    const int j = TPR * jj + h;
+                    if (j >= c0 && j < c1) L.U[pos - c0][j - c0] = a[jj];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (h == 0)
+#pragma unroll
+                for (int q = 0; q < kNrhsMax; ++q) L.y[pos - c0][q] = b[q];
+        }
+        sync();
+        // (b) one wave solves the block: lane i holds row i
+        if (wig == 0 && c1 > 0)
+        {
+            const int bs = c1 - c0;
+            float yv[kNrhsMax];
+#pragma unroll
+            for (int q = 0; q < kNrhsMax; ++q) yv[q] = lane < bs ? L.y[lane][q] : 0.0f;
+            for (int s2 = bs - 1; s2 >= 0; --s2)
+            {
+                const float dss = L.U[s2][s2];
+                const float u = lane < s2 ? L.U[lane][s2] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < kNrhsMax; ++q)
+                {
+                    const float xs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(yv[q]), s2)) / dss;
+                    if (lane == s2) yv[q] = xs;
+                    yv[q] = fmaf(-u, xs, yv[q]);
+                    if (lane == s2) yv[q] = xs;
+                }
+            }
+            if (lane < bs)
+#pragma unroll
+                for (int q = 0; q < kNrhsMax; ++q) L.y[lane][q] = yv[q];
+        }
+        sync();
+        // (c) rows of the block take their solution; rows above fold the block in
+        if (inBlk)
+        {
+#pragma unroll
+            for (int q = 0; q < kNrhsMax; ++q) b[q] = L.y[pos - c0][q];
+        }
+        float part[kNrhsMax] = {};
+        const bool above = rowLive && pos < c0;
+        if (above)
+        {
+#pragma unroll
+            for (int cc = 0; cc < C; cc += CH)
+            {
+                if (TPR * (cc + CH) <= c0 || TPR * cc >= c1) continue;  // uniform
+#pragma unroll
+                for (int jj = cc; jj < cc + CH; ++jj)
+                {
+                    const int j = TPR * jj + h;
+                    if (j >= c0 && j < c1)
+#pragma unroll
+                        for (int q = 0; q < kNrhsMax; ++q) part[q] = fmaf(a[jj], L.y[j - c0][q], part[q]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (TPR > 1)
         {
+            if (above && h == 1)
+#pragma unroll
+                for (int q = 0; q < kNrhsMax; ++q) L.part[r][q] = part[q];
             sync();
-            if (active && !ownK) l = s_l[r];
-        }
-        if (active)
-        {
-            float next = 0.0f;
+            if (above && h == 0)
 #pragma unroll
-            for (int jj = 0; jj < C; ++jj)
-            {
-                const int j = TPR * jj + h;
-                if (j > k)
-                {
-                    a[jj] = fmaf(-l, s_prow[j], a[jj]);
-                    if (j == k + 1) next = a[jj];
-                }
-            }
-            if (rhsOwner)
-            {
-#pragma unroll
-                for (int qq = 0; qq < kNrhsMax; ++qq) b[qq] = fmaf(-l, s_rhs[qq], b[qq]);
-            }
-            cur = next;
+                for (int q = 0; q < kNrhsMax; ++q) part[q] += L.part[r][q];
         }
+        if (above && h == 0)
+#pragma unroll
+            for (int q = 0; q < kNrhsMax; ++q) b[q] -= part[q];
         sync();
     }
-    // Back substitution over positions: the row at position p holds U(p, p:n).
-    for (int k = NMAX - 1; k >= 0; --k)
-    {
-        if (WAVE && k >= steps) continue;
-        const bool active = k < steps;
-        const bool ownK = (k % TPR) == h;
-        float ukk = 0.0f;
-        if (active && ownK && pos != INT_MAX)
-        {
-#pragma unroll
-            for (int jj = 0; jj < C; ++jj)
-                if (TPR * jj + h == k) ukk = a[jj];
-            s_l[r] = ukk;  // U(pos, k) of this row
-        }
-        sync();
-        if (active && rhsOwner && pos == k)
-        {
-            const float dkk = s_l[r];
-#pragma unroll
-            for (int q = 0; q < kNrhsMax; ++q)
-            {
-                b[q] = b[q] / dkk;
-                s_rhs[q] = b[q];
-            }
-        }
-        sync();
-        if (active && rhsOwner && pos < k)
-        {
-            const float u = s_l[r];
-#pragma unroll
-            for (int q = 0; q < kNrhsMax; ++q) b[q] = fmaf(-u, s_rhs[q], b[q]);
-        }
-        sync();
-    }
+
     if (rowLive)
     {
+        float* dst = A + pos + static_cast<size_t>(h) * n;
+        const size_t stride = static_cast<size_t>(TPR) * n;
 #pragma unroll
-        for (int jj = 0; jj < C; ++jj)
+        for (int c0 = 0; c0 < C; c0 += 16)
         {
-            const int j = TPR * jj + h;
-            if (j < n) A[static_cast<size_t>(j) * n + pos] = a[jj];
+#pragma unroll
+            for (int jj = c0; jj < c0 + 16; ++jj)
+            {
+                const int j = TPR * jj + h;
+                if (j < n) dst[jj * stride] = a[jj];
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        if (rhsOwner)
-        {
+        if (h == 0)
 #pragma unroll
             for (int q = 0; q < kNrhsMax; ++q)
                 if (q < nrhs) B[static_cast<size_t>(q) * n + pos] = b[q];
-        }
     }
     if (live && t == 0) *info = zero;
 }
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
+
+union VbLds
+{
+    GroupLds<64, 1> s[8];
+    GroupLds<128, 1> m[4];
+    GroupLds<256, 2> l;
+};
 
 __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ items, int numItems,
                                                        const int* __restrict__ order, const int* __restrict__ nArr,
@@ -247,12 +366,7 @@ __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ 
                                                        int* __restrict__ ipiv, int* __restrict__ info,
                                                        int* __restrict__ counter)
 {
-    // Per-group LDS slices: group g of a class with NMAX rows uses [g * NMAX, (g+1) * NMAX).
-    __shared__ float s_prow[1024];
-    __shared__ float s_l[1024];
-    __shared__ float s_rhs[16][kNrhsMax];
-    __shared__ float s_best[16][16];
-    __shared__ int s_bpos[16][16];
+    __shared__ VbLds lds;
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (;;)
@@ -263,29 +377,31 @@ __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ 
         __syncthreads();
         if (it >= numItems) break;  // every workgroup reaches this exit once the queue drains
         const Item item = items[it];
-        if (item.bin == 0)  // 16 matrices, one wave64 each, wave-level synchronization only
+        if (item.bin == 0)  // 8 matrices, one wave64 each, wave-level synchronization only
         {
             const int g = tid >> 6, t = tid & 63;
             const bool live = g < item.count;
             const int mi = live ? order[item.first + g] : 0;
-            solve_group<64, 1>(live ? nArr[mi] : 0, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
-                               ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, s_prow + g * 64, s_rhs[g],
-                               s_best[g], s_bpos[g], s_l + g * 64);
+            const int n = live ? nArr[mi] : 0;
+            solve_group<64, 1, true>(n, n, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
+                                     ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, lds.s[g]);
         }
-        else if (item.bin == 1)  // 4 matrices, 256 threads each (two threads per row)
+        else if (item.bin == 1)  // 4 matrices, 128 threads each, in lockstep (largest first)
         {
-            const int g = tid >> 8, t = tid & 255;
+            const int g = tid >> 7, t = tid & 127;
             const bool live = g < item.count;
             const int mi = live ? order[item.first + g] : 0;
-            solve_group<128, 2>(live ? nArr[mi] : 0, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
-                                ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, s_prow + g * 128, s_rhs[g],
-                                s_best[g], s_bpos[g], s_l + g * 128);
+            const int nLead = nArr[order[item.first]];
+            solve_group<128, 1, false>(live ? nArr[mi] : 0, nLead, nrhs, live ? Aarr[mi] : nullptr,
+                                       live ? Barr[mi] : nullptr, ipiv + (live ? pivOff[mi] : 0), info + mi, t, live,
+                                       lds.m[g]);
         }
-        else  // 1 matrix, four threads per row
+        else  // 1 matrix, two threads per row
         {
             const int mi = order[item.first];
-            solve_group<256, 4>(nArr[mi], nrhs, Aarr[mi], Barr[mi], ipiv + pivOff[mi], info + mi, tid, true, s_prow,
-                                s_rhs[0], s_best[0], s_bpos[0], s_l);
+            const int n = nArr[mi];
+            solve_group<256, 2, false>(n, n, nrhs, Aarr[mi], Barr[mi], ipiv + pivOff[mi], info + mi, tid, true,
+                                       lds.l);
         }
         __syncthreads();
     }
@@ -425,7 +541,7 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
     {
         const int n = ns[order[i]];
         const int bin = n <= 64 ? 0 : (n <= 128 ? 1 : 2);
-        const int per = bin == 0 ? 16 : (bin == 1 ? 4 : 1);
+        const int per = bin == 0 ? 8 : (bin == 1 ? 4 : 1);
         int cnt = 0;
         while (cnt < per && i + cnt < batchSize)
         {
