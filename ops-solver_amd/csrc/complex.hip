@@ -393,56 +393,56 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int M, int N, int K, con
     }
 }
 
-// B <- op(T)^{-1} B on one diagonal block of at most 64 rows, complex; op: 0 = T, 1 = T^T,
-// 2 = T^H. Every thread owns one column of B in registers and runs the substitution on it
-// alone; the block of op(T) (and the FP64 reciprocals of its diagonal) sit in LDS and are
-// read by all lanes at the same address. Per element: the same SubMul step per term and
-// the same FP64-reciprocal scaling per pivot as the unblocked complex solve.
+// B <- op(T)^{-1} B on one diagonal block of kb <= 64 rows, complex; op: 0 = T, 1 = T^T,
+// 2 = T^H. Every thread owns one column of B in registers, organized as a shift register
+// (the unknown of step t is x[0]; the update x[i] = x[i+1] - T x[0] shifts the rest down),
+// so the loop body is shared by all steps and the kernel stays small. The block of op(T)
+// in step order and the FP64 reciprocals of its diagonal sit in LDS. Per element: the
+// same SubMul step per term and the same FP64-reciprocal scaling per pivot as the
+// unblocked complex solve.
 template <bool FWD, bool UNIT, int TRANS>
 __global__ __launch_bounds__(256) void ctrsm_diag64_kernel(int kb, int N, const float2* __restrict__ T, int ldt,
                                                            float2* __restrict__ B, int ldb)
 {
-    __shared__ float2 Ts[64][64];  // Ts[s][i] = op(T)(i, s)
+    __shared__ float2 Tsh[64][64];
     __shared__ double2 Dinv[64];
+    auto opT = [&](int i, int s2)
+    { return Conj(TRANS ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i], TRANS == 2); };
     for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        const int s = e >> 6, i = e & 63;
+        const int t = e >> 6, ip = e & 63;
         float2 v = make_float2(0.0f, 0.0f);
-        if (i < kb && s < kb)
-            v = Conj(TRANS ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i], TRANS == 2);
-        Ts[s][i] = v;
+        if (t < kb)
+        {
+            const int s2 = FWD ? t : kb - 1 - t;
+            const int i = FWD ? t + 1 + ip : s2 - 1 - ip;
+            if (i >= 0 && i < kb) v = opT(i, s2);
+        }
+        Tsh[t][ip] = v;
     }
     if (!UNIT)
-        for (int s = threadIdx.x; s < kb; s += 256) Dinv[s] = Recip64(Conj(T[static_cast<size_t>(s) * ldt + s], TRANS == 2));
+        for (int t = threadIdx.x; t < kb; t += 256)
+        {
+            const int s2 = FWD ? t : kb - 1 - t;
+            Dinv[t] = Recip64(opT(s2, s2));
+        }
     __syncthreads();
     const int col = blockIdx.x * 256 + threadIdx.x;
     if (col >= N) return;
     float2* b = B + static_cast<size_t>(col) * ldb;
     float2 x[64];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[i] : make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (int t = 0; t < 64; ++t)
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : make_float2(0.0f, 0.0f);
+#pragma unroll 1
+    for (int t = 0; t < kb; ++t)
     {
-        const int s = FWD ? t : 63 - t;
-        if (s < kb)
-        {
-            if (!UNIT) x[s] = Mul64(x[s], Dinv[s]);
-            if (FWD)
-            {
+        float2 xs = x[0];
+        if (!UNIT) xs = Mul64(xs, Dinv[t]);
+        b[FWD ? t : kb - 1 - t] = xs;
 #pragma unroll
-                for (int i = s + 1; i < 64; ++i) x[i] = SubMul(x[i], Ts[s][i], x[s]);
-            }
-            else
-            {
-#pragma unroll
-                for (int i = 0; i < s; ++i) x[i] = SubMul(x[i], Ts[s][i], x[s]);
-            }
-        }
+        for (int i = 0; i < 63; ++i) x[i] = SubMul(x[i + 1], Tsh[t][i], xs);
+        x[63] = make_float2(0.0f, 0.0f);
     }
-#pragma unroll
-    for (int i = 0; i < 64; ++i)
-        if (i < kb) b[i] = x[i];
 }
 
 hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, const float2* B, int ldb, float2* C,

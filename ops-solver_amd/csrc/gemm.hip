@@ -426,52 +426,60 @@ __global__ __launch_bounds__(256) void gemm_sub_bf16x3_kernel(int M, int N, int 
         }
 }
 
-// B <- op(T)^{-1} B on one diagonal block of at most 64 rows. Every thread owns one column
-// of B in registers and runs the substitution on it alone (no barriers, no exchange);
-// the block of op(T) sits in LDS, read by all lanes at the same address (broadcast). The
-// update order per element is the LAPACK strsm order: ascending (forward) or descending
-// (backward) pivot index, one fused multiply-add per term, one division per pivot.
+// B <- op(T)^{-1} B on one diagonal block of kb <= 64 rows. Every thread owns one column of
+// B in registers and runs the substitution on it alone (no barriers, no exchange). The
+// registers form a shift register: the unknown solved at step t is always x[0] and the
+// update x[i] = x[i+1] - T(., .) x[0] shifts the rest down, so the loop body is the same
+// at every step and the kernel stays a few KiB (a fully unrolled triangle is ~30 KiB of
+// straight-line code, which a single-workgroup launch fetches cold). op(T) sits in LDS in
+// step order and is read by all lanes at the same address. Per element the update order
+// is the LAPACK strsm order: ascending (forward) or descending (backward) pivot index,
+// one fused multiply-add per term, one division per pivot.
 template <bool FWD, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const float* __restrict__ T, int ldt,
                                                           int transT, float* __restrict__ B, int ldb)
 {
-    __shared__ __attribute__((aligned(16))) float Ts[64][64];  // Ts[s][i] = op(T)(i, s)
+    // step t solves row s = t (forward) or s = kb-1-t (backward); Tsh[t][i'] is the
+    // coefficient of x_s in the row that sits at shift-register slot i' after that step.
+    __shared__ __attribute__((aligned(16))) float Tsh[64][64];
+    __shared__ float Dg[64];
+    auto opT = [&](int i, int s2)
+    { return transT ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i]; };
     for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        const int s = e >> 6, i = e & 63;
+        const int t = e >> 6, ip = e & 63;
         float v = 0.0f;
-        if (i < kb && s < kb) v = transT ? T[static_cast<size_t>(i) * ldt + s] : T[static_cast<size_t>(s) * ldt + i];
-        Ts[s][i] = v;
+        if (t < kb)
+        {
+            const int s2 = FWD ? t : kb - 1 - t;
+            const int i = FWD ? t + 1 + ip : s2 - 1 - ip;
+            if (i >= 0 && i < kb) v = opT(i, s2);
+        }
+        Tsh[t][ip] = v;
     }
+    if (!UNIT)
+        for (int t = threadIdx.x; t < kb; t += 256)
+        {
+            const int s2 = FWD ? t : kb - 1 - t;
+            Dg[t] = opT(s2, s2);
+        }
     __syncthreads();
     const int col = blockIdx.x * 256 + threadIdx.x;
     if (col >= N) return;
     float* b = B + static_cast<size_t>(col) * ldb;
     float x[64];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[i] : 0.0f;
-#pragma unroll
-    for (int t = 0; t < 64; ++t)
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : 0.0f;
+#pragma unroll 1
+    for (int t = 0; t < kb; ++t)
     {
-        const int s = FWD ? t : 63 - t;
-        if (s < kb)
-        {
-            if (!UNIT) x[s] = x[s] / Ts[s][s];
-            if (FWD)
-            {
+        float xs = x[0];
+        if (!UNIT) xs = xs / Dg[t];
+        b[FWD ? t : kb - 1 - t] = xs;
 #pragma unroll
-                for (int i = s + 1; i < 64; ++i) x[i] = fmaf(-Ts[s][i], x[s], x[i]);
-            }
-            else
-            {
-#pragma unroll
-                for (int i = 0; i < s; ++i) x[i] = fmaf(-Ts[s][i], x[s], x[i]);
-            }
-        }
+        for (int i = 0; i < 63; ++i) x[i] = fmaf(-Tsh[t][i], xs, x[i + 1]);
+        x[63] = 0.0f;
     }
-#pragma unroll
-    for (int i = 0; i < 64; ++i)
-        if (i < kb) b[i] = x[i];
 }
 
 hipError_t TrsmDiag(int kb, int N, const float* T, int ldt, bool fwd, bool unit, bool transT, float* B, int ldb,
