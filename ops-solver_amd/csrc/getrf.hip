@@ -103,8 +103,15 @@ __device__ inline unsigned long long WaveMaxU64(unsigned long long v)
 // position -- the first-row rule of a physical swap. Per column: one 64-bit key argmax
 // per thread, a DPP wave argmax, and one barrier, after which every thread reads the
 // wave winners' keys and the chosen candidate's row from LDS (double-buffered by column
-// parity, so one barrier per column suffices). At the end each row is stored at its final
-// position, which applies the interchanges to the leaf columns.
+// parity, so one barrier per column suffices).
+//
+// Each row's W values form a shift register: slot 0 always holds the column being
+// processed and every step rotates the row by one slot, so one rolled loop body serves
+// all W columns (and the loads and stores rotate the same way). A leaf runs as a single
+// workgroup on whichever CU is free, with a cold instruction cache; keeping the kernel a
+// few KiB instead of tens of KiB of unrolled code is what keeps it short. At the end each
+// row is stored at its final position, which applies the interchanges to the leaf
+// columns; the interchanges are then applied to the other panel columns as well.
 // ---------------------------------------------------------------------------------------
 template <int T, int R, int W>
 __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
@@ -125,19 +132,41 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 
     float a[R][W];
     int pos[R];
+    auto rotate = [&]()
+    {
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+        {
+            const float t0 = a[i][0];
+#pragma unroll
+            for (int k = 0; k < W - 1; ++k) a[i][k] = a[i][k + 1];
+            a[i][W - 1] = t0;
+        }
+    };
+    // Load: column j enters at slot W-1; after W steps slot k holds column k.
+#pragma unroll 1
+    for (int j = 0; j < W; ++j)
+    {
+        const float* col = Ap + static_cast<size_t>(j) * lda;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+        {
+#pragma unroll
+            for (int k = 0; k < W - 1; ++k) a[i][k] = a[i][k + 1];
+            const int lr = tid + i * T;
+            a[i][W - 1] = (lr < mp && j < wact) ? col[lr] : 0.0f;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i)
     {
         const int lr = tid + i * T;
-        const bool valid = lr < mp;
-#pragma unroll
-        for (int j = 0; j < W; ++j) a[i][j] = (valid && j < wact) ? Ap[static_cast<size_t>(j) * lda + lr] : 0.0f;
-        pos[i] = valid ? lr : INT_MAX;
+        pos[i] = lr < mp ? lr : INT_MAX;
     }
     if (tid == 0) s_count = 0;
     int zeroCol = 0;
 
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < W; ++j)
     {
         if (j < wact)  // uniform
@@ -150,7 +179,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
             for (int i = 0; i < R; ++i)
             {
                 const bool cand = pivoting ? (pos[i] >= j && pos[i] != INT_MAX) : pos[i] == j;
-                const unsigned long long k = cand ? PivotKey(a[i][j], pos[i]) | (1ull << 63) : 0ull;
+                const unsigned long long k = cand ? PivotKey(a[i][0], pos[i]) | (1ull << 63) : 0ull;
                 if (k > best)
                 {
                     best = k;
@@ -195,42 +224,58 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                     pos[i] = bpos;
             }
             if (tid == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
-            const float d = prow[j];
+            const float d = prow[0];
             if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
-            // Branch-free update: rows that are not below the pivot take l = 0, for which
-            // the FMA leaves them unchanged (a zero pivot skips the column likewise).
+            // Branch-free update of the remaining columns (slots 1 .. W-1-j; the finished
+            // columns sit in the last j slots and are left alone). Rows that are not below
+            // the pivot take l = 0, which leaves them unchanged, as does a zero pivot.
             const bool recip = fabsf(d) >= FLT_MIN;
             const float r = 1.0f / d;
+            const int live = W - j;  // slots [0, live) are unfinished
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
                 const bool below = pos[i] > j && pos[i] != INT_MAX && d != 0.0f;
-                const float q = recip ? a[i][j] * r : a[i][j] / d;
+                const float q = recip ? a[i][0] * r : a[i][0] / d;
                 const float l = below ? q : 0.0f;
-                a[i][j] = below ? q : a[i][j];
+                a[i][0] = below ? q : a[i][0];
 #pragma unroll
-                for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, prow[k], a[i][k]);
+                for (int k = 1; k < W; ++k)
+                {
+                    const float u = fmaf(-l, prow[k], a[i][k]);
+                    a[i][k] = k < live ? u : a[i][k];
+                }
             }
         }
+        rotate();
     }
 
-    // Store each row at its final position and record where the data came from.
+    // Store each row at its final position (slot 0 holds column j at step j) and record
+    // where the data came from.
+#pragma unroll 1
+    for (int j = 0; j < W; ++j)
+    {
+        if (j < wact)
+        {
+            float* col = Ap + static_cast<size_t>(j) * lda;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const int lr = tid + i * T;
+                if (lr < mp) col[pos[i]] = a[i][0];
+            }
+        }
+        rotate();
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i)
     {
         const int lr = tid + i * T;
-        if (lr < mp)
+        if (lr < mp && pos[i] != lr)
         {
-            const int dst = pos[i];
-#pragma unroll
-            for (int j = 0; j < W; ++j)
-                if (j < wact) Ap[static_cast<size_t>(j) * lda + dst] = a[i][j];
-            if (dst != lr)
-            {
-                const int slot = atomicAdd(&s_count, 1);
-                s_mdst[slot] = c0 + dst;
-                s_msrc[slot] = c0 + lr;
-            }
+            const int slot = atomicAdd(&s_count, 1);
+            s_mdst[slot] = c0 + pos[i];
+            s_msrc[slot] = c0 + lr;
         }
     }
     // Panel-level sources: read before any write (rows only exchange among themselves).
