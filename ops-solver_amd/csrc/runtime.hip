@@ -4,6 +4,8 @@
 // src/utils/real_lu_common.h:90-132 + src/sgetrf/sgetrf_kernel.cpp:98-152 (finite check).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <new>
 
 #include "internal.h"
@@ -195,6 +197,9 @@ const LookAhead* HandleLookAhead(aclsolverHandle_t handle)
     Handle* h = H(handle);
     if (h->laReady) return h->la.side ? &h->la : nullptr;
     h->laReady = true;
+    // OPS_SOLVER_LOOKAHEAD=0 keeps every factorization on the caller's stream (tuning aid).
+    const char* env = std::getenv("OPS_SOLVER_LOOKAHEAD");
+    if (env != nullptr && std::atoi(env) == 0) return nullptr;
     LookAhead la;
     bool ok = hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking) == hipSuccess;
     hipEvent_t* evs[5] = {&la.fork, &la.panel[0], &la.panel[1], &la.bulk[0], &la.bulk[1]};
