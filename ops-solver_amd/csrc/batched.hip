@@ -32,18 +32,52 @@ namespace ops
 namespace
 {
 
-// Max over an NL-lane group of (value) then min over (key) among lanes holding the max.
-template <int NL>
-__device__ inline void GroupArgmax(float val, int key, float* bestVal, int* bestKey)
+// Pivot key: larger |a| first, then the lower (position, lane); NaN never wins (|a| bits + 1
+// in the high word, 0 for NaN), non-candidates are 0.
+__device__ inline unsigned long long GjKey(float v, int posLane)
 {
-    float b = val;
-#pragma unroll
-    for (int off = NL / 2; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off));
-    int k = (val == b) ? key : INT_MAX;
-#pragma unroll
-    for (int off = NL / 2; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off));
-    *bestVal = b;
-    *bestKey = k;
+    const float a = fabsf(v);
+    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;
+    return (1ull << 63) | (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~posLane);
+}
+
+template <int CTRL>
+__device__ inline unsigned long long GjDpp(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xffffffffull), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, 0xF, 0xF, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+
+// Maximum key over each aligned NL-lane group (NL = 8, 16, 32 or 64), left in every lane
+// of the group: DPP quad swaps and (half-)row mirrors within 16-lane rows, a swizzle across
+// the two rows of a 32-lane half, lane reads across the halves.
+template <int NL>
+__device__ inline unsigned long long GroupMaxKey(unsigned long long v)
+{
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    v = mx(v, GjDpp<0xB1>(v));                  // quad_perm [1,0,3,2]
+    v = mx(v, GjDpp<0x4E>(v));                  // quad_perm [2,3,0,1]
+    v = mx(v, GjDpp<0x141>(v));                 // row_half_mirror -> groups of 8
+    if (NL >= 16) v = mx(v, GjDpp<0x140>(v));   // row_mirror -> rows of 16
+    if (NL >= 32)
+    {
+        // xor 16 inside each 32-lane half (swizzle bit-mask mode: and 0x1f, xor 0x10)
+        const int lo = __builtin_amdgcn_ds_swizzle(static_cast<int>(v & 0xffffffffull), 0x401F);
+        const int hi = __builtin_amdgcn_ds_swizzle(static_cast<int>(v >> 32), 0x401F);
+        v = mx(v, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo));
+    }
+    if (NL >= 64)
+    {
+        auto rl = [](unsigned long long x, int lane)
+        {
+            const unsigned l = __builtin_amdgcn_readlane(static_cast<int>(x & 0xffffffffull), lane);
+            const unsigned h = __builtin_amdgcn_readlane(static_cast<int>(x >> 32), lane);
+            return (static_cast<unsigned long long>(h) << 32) | l;
+        };
+        v = mx(rl(v, 0), rl(v, 32));
+    }
+    return v;
 }
 
 // MATS matrices per workgroup of 256 threads (4 waves); each NL-lane group one matrix.
@@ -82,10 +116,10 @@ __global__ __launch_bounds__(256) void matinv_gj_kernel(int n, const float* cons
     for (int k = 0; k < NL; ++k)
     {
         if (k >= n) continue;
-        const float cand = (rowLive && !used) ? fabsf(S[k]) : -1.0f;
-        float best;
-        int key;
-        GroupArgmax<NL>(cand, (pos << 8) | lane, &best, &key);
+        const unsigned long long myKey = (rowLive && !used) ? GjKey(S[k], (pos << 8) | lane) : 0ull;
+        const unsigned long long top = GroupMaxKey<NL>(myKey);
+        // No candidate at all cannot happen for k < n (position k's row is a candidate).
+        const int key = static_cast<int>(~static_cast<unsigned>(top & 0xffffffffull));
         const int p = key & 0xFF;
         const int ppos = key >> 8;
         const bool isP = lane == p;

@@ -214,15 +214,27 @@ __device__ __attribute__((always_inline)) void solve_group(int n, int nLoop, int
 #pragma unroll
             for (int c0 = 0; c0 < C; c0 += CH)
             {
-                if (TPR * (c0 + CH) <= k) continue;  // uniform: every column of the chunk is < k
-#pragma unroll
-                for (int jj = c0; jj < c0 + CH; ++jj)
+                // Columns of this thread in the chunk: TPR*c0 + h .. TPR*(c0+CH-1) + h
+                // (h is uniform per wave, k per group: every branch below is uniform).
+                const int jlo = TPR * c0 + h, jhi = TPR * (c0 + CH - 1) + h;
+                if (jhi < k) continue;  // finished columns
+                if (jlo > k + 1)
                 {
-                    const int j = TPR * jj + h;
-                    const float u = L.prow[h][jj];
-                    if (j > k) a[jj] = fmaf(-l, u, a[jj]);
-                    if (j == k && ownK && l != 0.0f) a[jj] = l;  // store the multiplier
-                    if (j == k + 1) next = a[jj];
+                    // all columns right of k + 1: plain rank-1 update
+#pragma unroll
+                    for (int jj = c0; jj < c0 + CH; ++jj) a[jj] = fmaf(-l, L.prow[h][jj], a[jj]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int jj = c0; jj < c0 + CH; ++jj)
+                    {
+                        const int j = TPR * jj + h;
+                        const float u = L.prow[h][jj];
+                        if (j > k) a[jj] = fmaf(-l, u, a[jj]);
+                        if (j == k && ownK && l != 0.0f) a[jj] = l;  // store the multiplier
+                        if (j == k + 1) next = a[jj];
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);  // keep each chunk's pivot-row loads local
             }
