@@ -103,15 +103,8 @@ __device__ inline unsigned long long WaveMaxU64(unsigned long long v)
 // position -- the first-row rule of a physical swap. Per column: one 64-bit key argmax
 // per thread, a DPP wave argmax, and one barrier, after which every thread reads the
 // wave winners' keys and the chosen candidate's row from LDS (double-buffered by column
-// parity, so one barrier per column suffices).
-//
-// Each row's W values form a shift register: slot 0 always holds the column being
-// processed and every step rotates the row by one slot, so one rolled loop body serves
-// all W columns (and the loads and stores rotate the same way). A leaf runs as a single
-// workgroup on whichever CU is free, with a cold instruction cache; keeping the kernel a
-// few KiB instead of tens of KiB of unrolled code is what keeps it short. At the end each
-// row is stored at its final position, which applies the interchanges to the leaf
-// columns; the interchanges are then applied to the other panel columns as well.
+// parity, so one barrier per column suffices). At the end each row is stored at its final
+// position, which applies the interchanges to the leaf columns.
 // ---------------------------------------------------------------------------------------
 template <int T, int R, int W>
 __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
@@ -132,41 +125,19 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 
     float a[R][W];
     int pos[R];
-    auto rotate = [&]()
-    {
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-        {
-            const float t0 = a[i][0];
-#pragma unroll
-            for (int k = 0; k < W - 1; ++k) a[i][k] = a[i][k + 1];
-            a[i][W - 1] = t0;
-        }
-    };
-    // Load: column j enters at slot W-1; after W steps slot k holds column k.
-#pragma unroll 1
-    for (int j = 0; j < W; ++j)
-    {
-        const float* col = Ap + static_cast<size_t>(j) * lda;
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-        {
-#pragma unroll
-            for (int k = 0; k < W - 1; ++k) a[i][k] = a[i][k + 1];
-            const int lr = tid + i * T;
-            a[i][W - 1] = (lr < mp && j < wact) ? col[lr] : 0.0f;
-        }
-    }
 #pragma unroll
     for (int i = 0; i < R; ++i)
     {
         const int lr = tid + i * T;
-        pos[i] = lr < mp ? lr : INT_MAX;
+        const bool valid = lr < mp;
+#pragma unroll
+        for (int j = 0; j < W; ++j) a[i][j] = (valid && j < wact) ? Ap[static_cast<size_t>(j) * lda + lr] : 0.0f;
+        pos[i] = valid ? lr : INT_MAX;
     }
     if (tid == 0) s_count = 0;
     int zeroCol = 0;
 
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < W; ++j)
     {
         if (j < wact)  // uniform
@@ -179,7 +150,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
             for (int i = 0; i < R; ++i)
             {
                 const bool cand = pivoting ? (pos[i] >= j && pos[i] != INT_MAX) : pos[i] == j;
-                const unsigned long long k = cand ? PivotKey(a[i][0], pos[i]) | (1ull << 63) : 0ull;
+                const unsigned long long k = cand ? PivotKey(a[i][j], pos[i]) | (1ull << 63) : 0ull;
                 if (k > best)
                 {
                     best = k;
@@ -224,58 +195,45 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                     pos[i] = bpos;
             }
             if (tid == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
-            const float d = prow[0];
+            const float d = prow[j];
             if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
-            // Branch-free update of the remaining columns (slots 1 .. W-1-j; the finished
-            // columns sit in the last j slots and are left alone). Rows that are not below
-            // the pivot take l = 0, which leaves them unchanged, as does a zero pivot.
-            const bool recip = fabsf(d) >= FLT_MIN;
+            // Branch-free update: rows that are not below the pivot take l = 0, for which
+            // the FMA leaves them unchanged (a zero pivot skips the column likewise). d is
+            // the same in every thread, so the reciprocal / division choice is a uniform
+            // branch (the division sequence only runs for a subnormal pivot).
+            const bool recip = fabsf(d) >= FLT_MIN;  // uniform
             const float r = 1.0f / d;
-            const int live = W - j;  // slots [0, live) are unfinished
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
                 const bool below = pos[i] > j && pos[i] != INT_MAX && d != 0.0f;
-                const float q = recip ? a[i][0] * r : a[i][0] / d;
+                float q = a[i][j] * r;
+                if (!recip) q = a[i][j] / d;  // uniform branch
                 const float l = below ? q : 0.0f;
-                a[i][0] = below ? q : a[i][0];
+                a[i][j] = below ? q : a[i][j];
 #pragma unroll
-                for (int k = 1; k < W; ++k)
-                {
-                    const float u = fmaf(-l, prow[k], a[i][k]);
-                    a[i][k] = k < live ? u : a[i][k];
-                }
+                for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, prow[k], a[i][k]);
             }
         }
-        rotate();
     }
 
-    // Store each row at its final position (slot 0 holds column j at step j) and record
-    // where the data came from.
-#pragma unroll 1
-    for (int j = 0; j < W; ++j)
-    {
-        if (j < wact)
-        {
-            float* col = Ap + static_cast<size_t>(j) * lda;
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-            {
-                const int lr = tid + i * T;
-                if (lr < mp) col[pos[i]] = a[i][0];
-            }
-        }
-        rotate();
-    }
+    // Store each row at its final position and record where the data came from.
 #pragma unroll
     for (int i = 0; i < R; ++i)
     {
         const int lr = tid + i * T;
-        if (lr < mp && pos[i] != lr)
+        if (lr < mp)
         {
-            const int slot = atomicAdd(&s_count, 1);
-            s_mdst[slot] = c0 + pos[i];
-            s_msrc[slot] = c0 + lr;
+            const int dst = pos[i];
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < wact) Ap[static_cast<size_t>(j) * lda + dst] = a[i][j];
+            if (dst != lr)
+            {
+                const int slot = atomicAdd(&s_count, 1);
+                s_mdst[slot] = c0 + dst;
+                s_msrc[slot] = c0 + lr;
+            }
         }
     }
     // Panel-level sources: read before any write (rows only exchange among themselves).
@@ -302,21 +260,28 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
     const int cnt = s_count;
     const int left = c0 - j0, others = left + (panelEnd - c0 - wact);
     constexpr int CPB = T / MOVES;  // columns per batch
+    constexpr int SWB = 8;          // batches gathered per barrier
     const int e = tid % MOVES;
-    for (int b0 = 0; b0 < others; b0 += CPB)  // uniform trip count
+    for (int b0 = 0; b0 < others; b0 += CPB * SWB)  // uniform trip count
     {
-        const int q = b0 + tid / MOVES;
-        const bool act = q < others && e < cnt;
-        float* col = nullptr;
-        float v = 0.0f;
-        if (act)
+        float v[SWB];
+        int cc[SWB];
+#pragma unroll
+        for (int bb = 0; bb < SWB; ++bb)
         {
-            const int c = q < left ? j0 + q : c0 + wact + (q - left);
-            col = A + static_cast<size_t>(c) * lda;
-            v = col[s_msrc[e]];
+            const int q = b0 + bb * CPB + tid / MOVES;
+            cc[bb] = -1;
+            v[bb] = 0.0f;
+            if (q < others && e < cnt)
+            {
+                cc[bb] = q < left ? j0 + q : c0 + wact + (q - left);
+                v[bb] = A[static_cast<size_t>(cc[bb]) * lda + s_msrc[e]];
+            }
         }
         __syncthreads();
-        if (act) col[s_mdst[e]] = v;
+#pragma unroll
+        for (int bb = 0; bb < SWB; ++bb)
+            if (cc[bb] >= 0) A[static_cast<size_t>(cc[bb]) * lda + s_mdst[e]] = v[bb];
     }
 }
 
