@@ -401,17 +401,14 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int M, int N, int K, con
 // same SubMul step per term and the same FP64-reciprocal scaling per pivot as the
 // unblocked complex solve.
 template <bool FWD, bool UNIT, int TRANS>
-__global__ __launch_bounds__(128) void ctrsm_diag64_kernel(int kb, int N, const float2* __restrict__ T, int ldt,
+__global__ __launch_bounds__(256) void ctrsm_diag64_kernel(int kb, int N, const float2* __restrict__ T, int ldt,
                                                            float2* __restrict__ B, int ldb)
 {
     __shared__ float2 Tsh[64][64];
     __shared__ double2 Dinv[64];
-    // The workgroup's 128 columns of B pass through LDS so that global accesses move whole
-    // column segments (coalesced); each thread then takes its column from LDS.
-    __shared__ float2 Bs[64][129];
     auto opT = [&](int i, int s2)
     { return Conj(TRANS ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i], TRANS == 2); };
-    for (int e = threadIdx.x; e < 64 * 64; e += 128)
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
         const int t = e >> 6, ip = e & 63;
         float2 v = make_float2(0.0f, 0.0f);
@@ -424,38 +421,27 @@ __global__ __launch_bounds__(128) void ctrsm_diag64_kernel(int kb, int N, const 
         Tsh[t][ip] = v;
     }
     if (!UNIT)
-        for (int t = threadIdx.x; t < kb; t += 128)
+        for (int t = threadIdx.x; t < kb; t += 256)
         {
             const int s2 = FWD ? t : kb - 1 - t;
             Dinv[t] = Recip64(opT(s2, s2));
         }
-    const int col0 = blockIdx.x * 128;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = wave; c < 128; c += 2)
-    {
-        const int col = col0 + c;
-        Bs[lane][c] = (col < N && lane < kb) ? B[static_cast<size_t>(col) * ldb + lane] : make_float2(0.0f, 0.0f);
-    }
     __syncthreads();
-    const int tc = threadIdx.x;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= N) return;
+    float2* b = B + static_cast<size_t>(col) * ldb;
     float2 x[64];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = Bs[FWD ? i : (kb - 1 - i >= 0 ? kb - 1 - i : 63)][tc];
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : make_float2(0.0f, 0.0f);
 #pragma unroll 1
     for (int t = 0; t < kb; ++t)
     {
         float2 xs = x[0];
         if (!UNIT) xs = Mul64(xs, Dinv[t]);
-        Bs[FWD ? t : kb - 1 - t][tc] = xs;
+        b[FWD ? t : kb - 1 - t] = xs;
 #pragma unroll
         for (int i = 0; i < 63; ++i) x[i] = SubMul(x[i + 1], Tsh[t][i], xs);
         x[63] = make_float2(0.0f, 0.0f);
-    }
-    __syncthreads();
-    for (int c = wave; c < 128; c += 2)
-    {
-        const int col = col0 + c;
-        if (col < N && lane < kb) B[static_cast<size_t>(col) * ldb + lane] = Bs[lane][c];
     }
 }
 
@@ -477,7 +463,7 @@ hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, c
 template <bool FWD, bool UNIT, int TRANS>
 hipError_t LaunchCTrsmDiag(int kb, int N, const float2* T, int ldt, float2* B, int ldb, hipStream_t stream)
 {
-    hipLaunchKernelGGL((ctrsm_diag64_kernel<FWD, UNIT, TRANS>), dim3((N + 127) / 128), dim3(128), 0, stream, kb, N, T,
+    hipLaunchKernelGGL((ctrsm_diag64_kernel<FWD, UNIT, TRANS>), dim3((N + 255) / 256), dim3(256), 0, stream, kb, N, T,
                        ldt, B, ldb);
     return hipGetLastError();
 }

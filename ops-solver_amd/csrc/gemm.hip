@@ -443,9 +443,6 @@ __global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const f
     // coefficient of x_s in the row that sits at shift-register slot i' after that step.
     __shared__ __attribute__((aligned(16))) float Tsh[64][64];
     __shared__ float Dg[64];
-    // The workgroup's 256 columns of B pass through LDS: global accesses read and write
-    // whole column segments (coalesced), each thread then takes its column from LDS.
-    __shared__ float Bs[64][257];
     auto opT = [&](int i, int s2)
     { return transT ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i]; };
     for (int e = threadIdx.x; e < 64 * 64; e += 256)
@@ -466,34 +463,22 @@ __global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const f
             const int s2 = FWD ? t : kb - 1 - t;
             Dg[t] = opT(s2, s2);
         }
-    const int col0 = blockIdx.x * 256;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // wave w moves columns w, w+4, ...: lane = row within the column segment
-    for (int c = wave; c < 256; c += 4)
-    {
-        const int col = col0 + c;
-        Bs[lane][c] = (col < N && lane < kb) ? B[static_cast<size_t>(col) * ldb + lane] : 0.0f;
-    }
     __syncthreads();
-    const int tc = threadIdx.x;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= N) return;
+    float* b = B + static_cast<size_t>(col) * ldb;
     float x[64];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = Bs[FWD ? i : (kb - 1 - i >= 0 ? kb - 1 - i : 63)][tc];
+    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : 0.0f;
 #pragma unroll 1
     for (int t = 0; t < kb; ++t)
     {
         float xs = x[0];
         if (!UNIT) xs = xs / Dg[t];
-        Bs[FWD ? t : kb - 1 - t][tc] = xs;
+        b[FWD ? t : kb - 1 - t] = xs;
 #pragma unroll
         for (int i = 0; i < 63; ++i) x[i] = fmaf(-Tsh[t][i], xs, x[i + 1]);
         x[63] = 0.0f;
-    }
-    __syncthreads();
-    for (int c = wave; c < 256; c += 4)
-    {
-        const int col = col0 + c;
-        if (col < N && lane < kb) B[static_cast<size_t>(col) * ldb + lane] = Bs[lane][c];
     }
 }
 
