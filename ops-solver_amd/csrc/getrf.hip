@@ -57,17 +57,6 @@ inline Work Layout(void* base, int m)
 }
 inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
 
-// Pivot key of a candidate: larger |a| first, then the lower position. The high word is
-// |a|'s bit pattern plus one (non-negative floats order like their bits) and 0 for NaN, so
-// a NaN never wins over a number and an all-NaN column keeps its diagonal position, as
-// with the comparison |a| > best; the low word is ~position.
-__device__ inline unsigned long long PivotKey(float v, int pos)
-{
-    const float a = fabsf(v);
-    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;
-    return (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~pos);
-}
-
 template <int CTRL>
 __device__ inline unsigned long long DppU64(unsigned long long v)
 {
@@ -132,7 +121,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
         const bool valid = lr < mp;
 #pragma unroll
         for (int j = 0; j < W; ++j) a[i][j] = (valid && j < wact) ? Ap[static_cast<size_t>(j) * lda + lr] : 0.0f;
-        pos[i] = valid ? lr : INT_MAX;
+        pos[i] = valid ? lr : -1;  // padding rows: never a candidate, never below a pivot
     }
     if (tid == 0) s_count = 0;
     int zeroCol = 0;
@@ -144,19 +133,29 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
         {
             const int par = j & 1;
             // Candidate of this thread: its best row (or, without pivoting, the row at j).
-            unsigned long long best = 0ull;
-            int bi = -1;
+            // First the largest |a| over the candidate rows (fmaxf skips NaN, so a NaN
+            // never wins), then the lowest position among the rows that reach it; only the
+            // thread's winner builds a 64-bit key for the cross-thread argmax.
+            float tmax = -1.0f;
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
-                const bool cand = pivoting ? (pos[i] >= j && pos[i] != INT_MAX) : pos[i] == j;
-                const unsigned long long k = cand ? PivotKey(a[i][j], pos[i]) | (1ull << 63) : 0ull;
-                if (k > best)
-                {
-                    best = k;
-                    bi = i;
-                }
+                const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
+                tmax = fmaxf(tmax, cand ? fabsf(a[i][j]) : -1.0f);
             }
+            int tpos = INT_MAX, bi = -1;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
+                const bool hit = cand && fabsf(a[i][j]) == tmax && pos[i] < tpos;
+                tpos = hit ? pos[i] : tpos;
+                bi = hit ? i : bi;
+            }
+            const unsigned long long best =
+                bi >= 0 ? (1ull << 63) | (static_cast<unsigned long long>(__float_as_uint(tmax) + 1u) << 32) |
+                              static_cast<unsigned>(~tpos)
+                        : 0ull;
             const unsigned long long wmax = WaveMaxU64(best);
             if (best == wmax && bi >= 0)  // the wave's winner (keys are unique per position)
             {
@@ -206,7 +205,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
-                const bool below = pos[i] > j && pos[i] != INT_MAX && d != 0.0f;
+                const bool below = pos[i] > j && d != 0.0f;
                 float q = a[i][j] * r;
                 if (!recip) q = a[i][j] / d;  // uniform branch
                 const float l = below ? q : 0.0f;
