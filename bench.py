@@ -37,9 +37,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_system(torch, n, seed):
+def make_system(torch, n, seed, kind="qr"):
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
+    if kind == "shifted":
+        # N(0,1)/sqrt(n) + 2I: well conditioned, no QR; the LU runs the same launch shapes
+        # (used for the counter passes, which slow every dispatch of the QR down).
+        at = torch.randn(n, n, device="cuda", generator=g) / n ** 0.5
+        at.diagonal().add_(2.0)
+        x_true = torch.randn(n, device="cuda", generator=g, dtype=torch.float64)
+        b = (at.T.double() @ x_true).float().contiguous()
+        return at, b, x_true
     G1 = torch.randn(n, n, device="cuda", generator=g)
     Q1, _ = torch.linalg.qr(G1)
     del G1
@@ -55,6 +63,21 @@ def make_system(torch, n, seed):
     b64 = at.T.double() @ x_true
     b = b64.float().contiguous()
     return at, b, x_true
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "gemm_packed_bf16x6_n16384_traffic.md")
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the packed trailing-update GEMM at n=16384, from the committed
+    FETCH_SIZE / WRITE_SIZE counter summary (scaled by the calibration copy); None otherwise."""
+    if n != 16384 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
 
 
 def errors(torch, at, x, b, x_true):
@@ -242,6 +265,10 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--modes", default="SS,SB", help="gesv variants to time: SS, SH, SB (SB, the headline, always runs)")
     ap.add_argument("--restart", type=int, default=60)
+    ap.add_argument("--matrix", choices=["qr", "shifted"], default="qr",
+                    help="qr: Q1 diag(s) Q2^T with cond 1e6 (the workload); shifted: N(0,1)/sqrt(n) + 2I")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 1 GiB device copy (the reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (block-cyclic / batched sharding) "
@@ -253,8 +280,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = args.n
-    log(f"generating A = Q1 diag(s) Q2^T, n={n}")
-    at, b, x_true = make_system(torch, n, seed=16384)
+    if args.copy_calibration:
+        src = torch.ones(256 * 1024 * 1024, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # 1 GiB read + 1 GiB write
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy done (1 GiB)")
+    log(f"generating A ({args.matrix}), n={n}")
+    at, b, x_true = make_system(torch, n, seed=16384, kind=args.matrix)
     torch.cuda.synchronize()
     h = A.Handle()
     A.lib().aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 20, args.restart)
@@ -296,12 +330,16 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": f"dense solve n={n}, A=Q1 diag(s) Q2^T (s log-spaced 1..1e6, cond 1e6), 1 RHS; "
-                               "BF16 MFMA LU (FP32 accumulate) + GMRES-IR with FP64 residual to bwd err 1e-6",
+        "config": {"workload": f"dense solve n={n}, " + (
+                       "A=Q1 diag(s) Q2^T (s log-spaced 1..1e6, cond 1e6)" if args.matrix == "qr" else
+                       "A=N(0,1)/sqrt(n)+2I") + ", 1 RHS; BF16 MFMA LU (FP32 accumulate) + GMRES-IR with FP64 "
+                       "residual to bwd err 1e-6",
                    "n": n, "nrhs": 1, "parallelism": "1 GPU"},
-        "roofline": {"bound": "mfma", "kernel": "trailing-update GEMM (bf16 MFMA, fp32 acc)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "gemm_packed_kernel<bf16,3>: trailing update, 3-part bf16 operand split "
+                               "(6 MFMA products per algorithmic FMA), fp32 accumulate",
                      "achieved": round(achieved / 1e12, 2), "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_BF16, 4), "traffic": measured_traffic(n),
                      "launches": launches, "avg_launch_ms": round(gms / max(launches, 1), 4),
                      "avg_launch_gflop": round(gflop / max(launches, 1) / 1e9, 3)},
         "backward_error": sb["backward_error"],

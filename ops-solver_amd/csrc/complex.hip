@@ -393,56 +393,71 @@ __global__ __launch_bounds__(256) void cgemm_sub_kernel(int M, int N, int K, con
     }
 }
 
+template <int CTRL>
+__device__ inline float2 QuadBroadcast2(float2 v)
+{
+    return make_float2(__int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), CTRL, 0xF, 0xF, false)),
+                       __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), CTRL, 0xF, 0xF, false)));
+}
+
 // B <- op(T)^{-1} B on one diagonal block of kb <= 64 rows, complex; op: 0 = T, 1 = T^T,
-// 2 = T^H. Every thread owns one column of B in registers, organized as a shift register
-// (the unknown of step t is x[0]; the update x[i] = x[i+1] - T x[0] shifts the rest down),
-// so the loop body is shared by all steps and the kernel stays small. The block of op(T)
-// in step order and the FP64 reciprocals of its diagonal sit in LDS. Per element: the
-// same SubMul step per term and the same FP64-reciprocal scaling per pivot as the
-// unblocked complex solve.
+// 2 = T^H. Four lanes (a quad) share one column of B, lane q holding rows 16q .. 16q+15 in
+// registers; the unknown of step s is broadcast in the quad with DPP, so a lane's serial
+// chain is 16 updates per step. op(T) and the FP64 reciprocals of its diagonal sit in LDS.
+// Per element: the same SubMul step per term, in pivot order, and the same FP64-reciprocal
+// scaling per pivot as the unblocked complex solve; rows outside the triangle are kept by
+// a select.
 template <bool FWD, bool UNIT, int TRANS>
 __global__ __launch_bounds__(256) void ctrsm_diag64_kernel(int kb, int N, const float2* __restrict__ T, int ldt,
                                                            float2* __restrict__ B, int ldb)
 {
-    __shared__ float2 Tsh[64][64];
+    __shared__ float2 Ts[64][64];  // Ts[s][i] = op(T)(i, s)
     __shared__ double2 Dinv[64];
     auto opT = [&](int i, int s2)
     { return Conj(TRANS ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i], TRANS == 2); };
     for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        const int t = e >> 6, ip = e & 63;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (t < kb)
-        {
-            const int s2 = FWD ? t : kb - 1 - t;
-            const int i = FWD ? t + 1 + ip : s2 - 1 - ip;
-            if (i >= 0 && i < kb) v = opT(i, s2);
-        }
-        Tsh[t][ip] = v;
+        const int s2 = e >> 6, i = e & 63;
+        Ts[s2][i] = (i < kb && s2 < kb) ? opT(i, s2) : make_float2(0.0f, 0.0f);
     }
     if (!UNIT)
-        for (int t = threadIdx.x; t < kb; t += 256)
-        {
-            const int s2 = FWD ? t : kb - 1 - t;
-            Dinv[t] = Recip64(opT(s2, s2));
-        }
+        for (int t = threadIdx.x; t < kb; t += 256) Dinv[t] = Recip64(opT(t, t));
     __syncthreads();
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= N) return;
-    float2* b = B + static_cast<size_t>(col) * ldb;
-    float2 x[64];
+    const int col = blockIdx.x * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+    const bool live = col < N;
+    float2* b = B + static_cast<size_t>(live ? col : 0) * ldb;
+    float2 x[16];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : make_float2(0.0f, 0.0f);
-#pragma unroll 1
-    for (int t = 0; t < kb; ++t)
+    for (int t = 0; t < 16; ++t) x[t] = (live && 16 * q + t < kb) ? b[16 * q + t] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < 64; ++u)
     {
-        float2 xs = x[0];
-        if (!UNIT) xs = Mul64(xs, Dinv[t]);
-        b[FWD ? t : kb - 1 - t] = xs;
+        const int s2 = FWD ? u : 63 - u;
+        if (s2 >= kb) continue;  // uniform
+        const int o = s2 >> 4, so = s2 & 15;
+        float2 xs;
+        if (o == 0)
+            xs = QuadBroadcast2<0x00>(x[so]);
+        else if (o == 1)
+            xs = QuadBroadcast2<0x55>(x[so]);
+        else if (o == 2)
+            xs = QuadBroadcast2<0xAA>(x[so]);
+        else
+            xs = QuadBroadcast2<0xFF>(x[so]);
+        if (!UNIT) xs = Mul64(xs, Dinv[s2]);
+        if (q == o) x[so] = xs;
 #pragma unroll
-        for (int i = 0; i < 63; ++i) x[i] = SubMul(x[i + 1], Tsh[t][i], xs);
-        x[63] = make_float2(0.0f, 0.0f);
+        for (int t = 0; t < 16; ++t)
+        {
+            const int row = 16 * q + t;
+            const bool upd = FWD ? row > s2 : row < s2;
+            const float2 nx = SubMul(x[t], Ts[s2][row], xs);
+            x[t] = upd ? nx : x[t];
+        }
     }
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if (live && 16 * q + t < kb) b[16 * q + t] = x[t];
 }
 
 hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, const float2* B, int ldb, float2* C,
@@ -463,7 +478,7 @@ hipError_t CGemmSub(int M, int N, int K, const float2* A, int lda, int transA, c
 template <bool FWD, bool UNIT, int TRANS>
 hipError_t LaunchCTrsmDiag(int kb, int N, const float2* T, int ldt, float2* B, int ldb, hipStream_t stream)
 {
-    hipLaunchKernelGGL((ctrsm_diag64_kernel<FWD, UNIT, TRANS>), dim3((N + 255) / 256), dim3(256), 0, stream, kb, N, T,
+    hipLaunchKernelGGL((ctrsm_diag64_kernel<FWD, UNIT, TRANS>), dim3((N + 63) / 64), dim3(256), 0, stream, kb, N, T,
                        ldt, B, ldb);
     return hipGetLastError();
 }

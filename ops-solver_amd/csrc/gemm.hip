@@ -426,66 +426,74 @@ __global__ __launch_bounds__(256) void gemm_sub_bf16x3_kernel(int M, int N, int 
         }
 }
 
-// B <- op(T)^{-1} B on one diagonal block of kb <= 64 rows. Every thread owns one column of
-// B in registers and runs the substitution on it alone (no barriers, no exchange). The
-// registers form a shift register: the unknown solved at step t is always x[0] and the
-// update x[i] = x[i+1] - T(., .) x[0] shifts the rest down, so the loop body is the same
-// at every step and the kernel stays a few KiB (a fully unrolled triangle is ~30 KiB of
-// straight-line code, which a single-workgroup launch fetches cold). op(T) sits in LDS in
-// step order and is read by all lanes at the same address. Per element the update order
-// is the LAPACK strsm order: ascending (forward) or descending (backward) pivot index,
-// one fused multiply-add per term, one division per pivot.
+template <int CTRL>
+__device__ inline float QuadBroadcast(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// B <- op(T)^{-1} B on one diagonal block of kb <= 64 rows. Four lanes (a quad) share one
+// column of B, lane q holding rows 16q .. 16q+15 in registers; the unknown solved at step s
+// is broadcast inside the quad with DPP, so each lane's serial chain is 64 steps of 16
+// multiply-adds instead of 64 x 63 for a column per lane. op(T) sits in LDS and is read by
+// all lanes of a step at the same column. Per element the update order is the LAPACK
+// strsm order: ascending (forward) or descending (backward) pivot index, one fused
+// multiply-add per term, one division per pivot; rows outside the triangle are left
+// untouched (a select, so a non-finite unknown cannot leak into finished rows).
 template <bool FWD, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const float* __restrict__ T, int ldt,
                                                           int transT, float* __restrict__ B, int ldb)
 {
-    // step t solves row s = t (forward) or s = kb-1-t (backward); Tsh[t][i'] is the
-    // coefficient of x_s in the row that sits at shift-register slot i' after that step.
-    __shared__ __attribute__((aligned(16))) float Tsh[64][64];
-    __shared__ float Dg[64];
-    auto opT = [&](int i, int s2)
-    { return transT ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i]; };
+    __shared__ __attribute__((aligned(16))) float Ts[64][64];  // Ts[s][i] = op(T)(i, s)
     for (int e = threadIdx.x; e < 64 * 64; e += 256)
     {
-        const int t = e >> 6, ip = e & 63;
+        const int s2 = e >> 6, i = e & 63;
         float v = 0.0f;
-        if (t < kb)
-        {
-            const int s2 = FWD ? t : kb - 1 - t;
-            const int i = FWD ? t + 1 + ip : s2 - 1 - ip;
-            if (i >= 0 && i < kb) v = opT(i, s2);
-        }
-        Tsh[t][ip] = v;
+        if (i < kb && s2 < kb) v = transT ? T[static_cast<size_t>(i) * ldt + s2] : T[static_cast<size_t>(s2) * ldt + i];
+        Ts[s2][i] = v;
     }
-    if (!UNIT)
-        for (int t = threadIdx.x; t < kb; t += 256)
-        {
-            const int s2 = FWD ? t : kb - 1 - t;
-            Dg[t] = opT(s2, s2);
-        }
     __syncthreads();
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= N) return;
-    float* b = B + static_cast<size_t>(col) * ldb;
-    float x[64];
+    const int col = blockIdx.x * 64 + (threadIdx.x >> 2), q = threadIdx.x & 3;
+    const bool live = col < N;
+    float* b = B + static_cast<size_t>(live ? col : 0) * ldb;
+    float x[16];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) x[i] = i < kb ? b[FWD ? i : kb - 1 - i] : 0.0f;
-#pragma unroll 1
-    for (int t = 0; t < kb; ++t)
+    for (int t = 0; t < 16; ++t) x[t] = (live && 16 * q + t < kb) ? b[16 * q + t] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 64; ++u)
     {
-        float xs = x[0];
-        if (!UNIT) xs = xs / Dg[t];
-        b[FWD ? t : kb - 1 - t] = xs;
+        const int s2 = FWD ? u : 63 - u;
+        if (s2 >= kb) continue;  // uniform
+        const int o = s2 >> 4, so = s2 & 15;
+        float xs;
+        if (o == 0)  // quad_perm [o,o,o,o]: every lane of the quad reads lane o
+            xs = QuadBroadcast<0x00>(x[so]);
+        else if (o == 1)
+            xs = QuadBroadcast<0x55>(x[so]);
+        else if (o == 2)
+            xs = QuadBroadcast<0xAA>(x[so]);
+        else
+            xs = QuadBroadcast<0xFF>(x[so]);
+        if (!UNIT) xs = xs / Ts[s2][s2];
+        if (q == o) x[so] = xs;
 #pragma unroll
-        for (int i = 0; i < 63; ++i) x[i] = fmaf(-Tsh[t][i], xs, x[i + 1]);
-        x[63] = 0.0f;
+        for (int t = 0; t < 16; ++t)
+        {
+            const int row = 16 * q + t;
+            const bool upd = FWD ? row > s2 : row < s2;
+            const float nx = fmaf(-Ts[s2][row], xs, x[t]);
+            x[t] = upd ? nx : x[t];
+        }
     }
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if (live && 16 * q + t < kb) b[16 * q + t] = x[t];
 }
 
 hipError_t TrsmDiag(int kb, int N, const float* T, int ldt, bool fwd, bool unit, bool transT, float* B, int ldb,
                     hipStream_t stream)
 {
-    const dim3 grid((N + 255) / 256);
+    const dim3 grid((N + 63) / 64);
     const int tr = transT ? 1 : 0;
     if (fwd && unit)
         hipLaunchKernelGGL((trsm_diag64_kernel<true, true>), grid, dim3(256), 0, stream, kb, N, T, ldt, tr, B, ldb);

@@ -323,9 +323,12 @@ hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const flo
                        p.KB, fa, SA, static_cast<Mfma<ET>::vec*>(ap));                                            \
     hipLaunchKernelGGL((pack_frag_kernel<ET, P>), dim3((fb * 64 + 255) / 256), dim3(256), 0, stream, N, K, B, lb, one, \
                        p.KB, fb, SB, static_cast<Mfma<ET>::vec*>(bp));                                            \
-    hipLaunchKernelGGL((gemm_packed_kernel<ET, P>), grid, dim3(256), 0, stream, M, N, p.KB,                       \
-                       static_cast<const Mfma<ET>::vec*>(ap), static_cast<const Mfma<ET>::vec*>(bp), C, ldc, tilesM, \
-                       tilesN, SA, SB)
+    {                                                                                                               \
+        ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * M * N * K, stream);                                    \
+        hipLaunchKernelGGL((gemm_packed_kernel<ET, P>), grid, dim3(256), 0, stream, M, N, p.KB,                     \
+                           static_cast<const Mfma<ET>::vec*>(ap), static_cast<const Mfma<ET>::vec*>(bp), C, ldc,    \
+                           tilesM, tilesN, SA, SB);                                                                 \
+    }
     if (prec == Precision::FP16X3)
     {
         // FP16 has 5 exponent bits: scale every row of A and column of B by a power of two.

@@ -321,7 +321,6 @@ hipError_t GetrfNoPivot(int n, float* a, int lda, int* info, Precision trailing,
         {
             GRET(TrsmBlock(jb, right, a + static_cast<size_t>(J) * lda + J, lda, true, true, false,
                            a + static_cast<size_t>(Jend) * lda + J, lda, s));
-            ProfileScope prof(ProfileKernel::TrailingGemm, 2.0 * right * right * jb, s);
             if (pack != nullptr)
                 GRET(GemmSubPacked(right, right, jb, a + static_cast<size_t>(J) * lda + Jend, lda,
                                    a + static_cast<size_t>(Jend) * lda + J, lda,

@@ -510,11 +510,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         const int nbNext = k + 1 < npanel ? std::min(NB, kmax - pe) : 0;
         // Bulk of the trailing update (s1): everything right of the next block column.
         if (two) RET_IF(hipStreamWaitEvent(s1, la->panel[k & 1], 0));
-        {
-            const double work2 = 2.0 * std::max(m - pe, 0) * std::max(n - pe - nbNext, 0) * std::min(NB, kmax - k * NB);
-            ProfileScope prof(ProfileKernel::TrailingGemm, work2, s1);
-            RET_IF(update(k, pe + nbNext, n, true, true, s1));
-        }
+        RET_IF(update(k, pe + nbNext, n, true, true, s1));
         if (two) RET_IF(hipEventRecord(la->bulk[k & 1], s1));
         // Critical path (s0): the next block column, then its panel.
         if (nbNext > 0)

@@ -77,7 +77,7 @@ bool AllFiniteArrays(aclsolverHandle_t h, const float* const* arrays, int rows, 
 // ---- optional launch timing (profile.hip) ----
 enum class ProfileKernel
 {
-    TrailingGemm = 0,
+    TrailingGemm = 0,  // the packed BF16/FP16 MFMA trailing-update kernel alone (2 M N K flops)
     BatchedInverse = 1,
     Vbatched = 2
 };

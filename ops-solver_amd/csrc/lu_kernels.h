@@ -23,7 +23,7 @@ hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const flo
 
 // Triangular solve with a K x K diagonal block T (column-major, ldt) on a K x N block B:
 // B <- op(T)^{-1} B. lower: T's lower triangle is used (else upper); unit: unit diagonal;
-// transT: op(T) = T^T. Runs as 64-row diagonal solves (one column per thread, in
+// transT: op(T) = T^T. Runs as 64-row diagonal solves (a quad of lanes per column, in
 // registers) interleaved with MFMA GEMM updates of the remaining rows.
 hipError_t TrsmBlock(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
                      hipStream_t stream);
