@@ -157,6 +157,75 @@ __global__ __launch_bounds__(256) void gemm_sub_f32_kernel(int M, int N, int K, 
     }
 }
 
+// FP32 update on 64 x 64 tiles (no transposes) for the skinny products of the panel
+// recursion, where N and K are 32 .. 256 and 128 x 128 tiles would leave most of the chip
+// idle: four waves hold one 32 x 32 accumulator each, k chunks of 16 double-buffered
+// through LDS as in gemm_sub_f32_kernel.
+__global__ __launch_bounds__(256) void gemm_sub_f32_64_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                              const float* __restrict__ B, int ldb,
+                                                              float* __restrict__ C, int ldc, int tilesM, int tilesN)
+{
+    constexpr int KC = 16, T64 = 64;
+    __shared__ float As[2][KC][T64 + 4];
+    __shared__ float Bs[2][KC][T64 + 4];
+    int tm, tn;
+    TileOf(tilesM, tilesN, &tm, &tn);
+    const int row0 = tm * T64, col0 = tn * T64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    // Per chunk each thread moves 4 elements of A (column k, row tid % 64) and 4 of B.
+    float ra[4], rb[4];
+    auto gload = [&](int k0)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const int r = tid & 63, k = (tid >> 6) * 4 + q;
+            ra[q] = (row0 + r < M && k0 + k < K) ? A[static_cast<size_t>(k0 + k) * lda + row0 + r] : 0.0f;
+            const int j = tid >> 2, kb = (tid & 3) * 4 + q;
+            rb[q] = (col0 + j < N && k0 + kb < K) ? B[static_cast<size_t>(col0 + j) * ldb + k0 + kb] : 0.0f;
+        }
+    };
+    auto lstore = [&](int buf)
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            As[buf][(tid >> 6) * 4 + q][tid & 63] = ra[q];
+            Bs[buf][(tid & 3) * 4 + q][tid >> 2] = rb[q];
+        }
+    };
+    const int chunks = (K + KC - 1) / KC;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int c = 0; c < chunks; ++c)
+    {
+        const int buf = c & 1;
+        if (c + 1 < chunks) gload((c + 1) * KC);
+#pragma unroll
+        for (int kk = 0; kk < KC; kk += 2)
+        {
+            const int k = kk + (lane >> 5);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Bs[buf][k][wn * 32 + (lane & 31)], As[buf][k][wm * 32 + (lane & 31)],
+                                                       acc, 0, 0, 0);
+        }
+        if (c + 1 < chunks) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    const int i = row0 + wm * 32 + (lane & 31);
+    if (i >= M) return;
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+    {
+        const int j = col0 + wn * 32 + (v & 3) + 8 * (v >> 2) + 4 * (lane >> 5);
+        if (j < N) C[static_cast<size_t>(j) * ldc + i] -= acc[v];
+    }
+}
+
 template <typename T>
 __device__ inline T ToHalf(float v);
 template <>
@@ -490,6 +559,137 @@ __global__ __launch_bounds__(256) void trsm_diag64_kernel(int kb, int N, const f
         if (live && 16 * q + t < kb) b[16 * q + t] = x[t];
 }
 
+// Inverses of the unit-lower 64 x 64 diagonal blocks of T (K x K): workgroup b writes
+// inv(T[64b.., 64b..]) to Linv + 4096 b (column-major, ld 64; rows and columns beyond the
+// block are the identity). Column c of the inverse is the forward substitution of e_c,
+// run like trsm_diag64_kernel: a quad of lanes per column, 16 rows each.
+__global__ __launch_bounds__(256) void trtri_unit_lower64_kernel(int K, const float* __restrict__ T, int ldt,
+                                                                 float* __restrict__ Linv)
+{
+    __shared__ __attribute__((aligned(16))) float Ts[64][64];  // Ts[s][i] = T(i, s) of the block
+    const int k0 = blockIdx.x * 64, kb = min(64, K - k0);
+    const float* Tb = T + static_cast<size_t>(k0) * ldt + k0;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
+    {
+        const int s2 = e >> 6, i = e & 63;
+        Ts[s2][i] = (i < kb && s2 < kb) ? Tb[static_cast<size_t>(s2) * ldt + i] : 0.0f;
+    }
+    __syncthreads();
+    const int col = threadIdx.x >> 2, q = threadIdx.x & 3;
+    float x[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = (16 * q + t == col) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int s2 = 0; s2 < 64; ++s2)
+    {
+        if (s2 >= kb) continue;  // uniform
+        const int o = s2 >> 4, so = s2 & 15;
+        float xs;
+        if (o == 0)
+            xs = QuadBroadcast<0x00>(x[so]);
+        else if (o == 1)
+            xs = QuadBroadcast<0x55>(x[so]);
+        else if (o == 2)
+            xs = QuadBroadcast<0xAA>(x[so]);
+        else
+            xs = QuadBroadcast<0xFF>(x[so]);
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+        {
+            const int row = 16 * q + t;
+            const float nx = fmaf(-Ts[s2][row], xs, x[t]);
+            x[t] = (row > s2 && row < kb) ? nx : x[t];
+        }
+    }
+    float* out = Linv + static_cast<size_t>(blockIdx.x) * 4096 + col * 64 + 16 * q;
+#pragma unroll
+    for (int t = 0; t < 16; t += 4)
+        *reinterpret_cast<float4*>(out + t) = make_float4(x[t], x[t + 1], x[t + 2], x[t + 3]);
+}
+
+// B <- T^{-1} B for unit-lower T (K x K, K <= 512) from the block inverses of
+// trtri_unit_lower64_kernel: one workgroup per 32-column strip of B, held in LDS; per
+// 64-row block b: X_b = inv(T_bb) B_b, then B_below -= T(below, b) X_b, both on FP32 MFMA
+// (32x32x2 tiles formed as products^T so that each accumulator lane holds one row).
+constexpr int kTrsmLuMaxK = 512;
+__global__ __launch_bounds__(256) void trsm_lu_kernel(int K, int N, const float* __restrict__ T, int ldt,
+                                                      const float* __restrict__ Linv, float* __restrict__ B, int ldb)
+{
+    __shared__ float Xs[kTrsmLuMaxK][33];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.x * 32, nc = min(32, N - c0);
+    const int Kp = (K + 63) & ~63;
+    for (int e = tid; e < 32 * Kp; e += 256)
+    {
+        const int c = e / Kp, r = e - c * Kp;
+        Xs[r][c] = (c < nc && r < K) ? B[static_cast<size_t>(c0 + c) * ldb + r] : 0.0f;
+    }
+    __syncthreads();
+    const int li = lane & 31, lk = lane >> 5;
+    const int nblk = Kp / 64;
+    // Operand fragments are loaded for a whole 32-row tile before its 32 MFMAs, so each tile
+    // waits for global memory once; the first tile below the block is fetched before the
+    // block's own product.
+    float lf[32];
+    auto loadTile = [&](int k0, int k1, int r0)
+    {
+        const int row = r0 + li;
+        const float* Lc = T + static_cast<size_t>(k0) * ldt + row;
+#pragma unroll
+        for (int t = 0; t < 32; ++t)
+        {
+            const int k = 2 * t + lk;
+            lf[t] = (row < K && k0 + k < k1) ? Lc[static_cast<size_t>(k) * ldt] : 0.0f;
+        }
+    };
+    for (int b = 0; b < nblk; ++b)
+    {
+        const int k0 = b * 64, k1 = min(K, k0 + 64);
+        float lb[32];
+        if (wave < 2)
+        {
+            const float* Lb = Linv + static_cast<size_t>(b) * 4096 + wave * 32 + li;
+#pragma unroll
+            for (int t = 0; t < 32; ++t) lb[t] = Lb[(2 * t + lk) * 64];
+        }
+        int r0 = k1 + wave * 32;
+        if (r0 < K) loadTile(k0, k1, r0);
+        // X_b = inv(T_bb) B_b: waves 0 and 1 take 32 rows each.
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+        if (wave < 2)
+#pragma unroll
+            for (int t = 0; t < 32; ++t)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Xs[k0 + 2 * t + lk][li], lb[t], acc, 0, 0, 0);
+        __syncthreads();
+        if (wave < 2)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) Xs[k0 + wave * 32 + li][(v & 3) + 8 * (v >> 2) + 4 * lk] = acc[v];
+        __syncthreads();
+        // Rows below the block: 32-row tiles dealt to the four waves.
+        for (; r0 < K; r0 += 128)
+        {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 32; ++t)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Xs[k0 + 2 * t + lk][li], lf[t], acc, 0, 0, 0);
+            const int row = r0 + li;
+            if (r0 + 128 < K) loadTile(k0, k1, r0 + 128);
+            if (row < K)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) Xs[row][(v & 3) + 8 * (v >> 2) + 4 * lk] -= acc[v];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 32 * Kp; e += 256)
+    {
+        const int c = e / Kp, r = e - c * Kp;
+        if (c < nc && r < K) B[static_cast<size_t>(c0 + c) * ldb + r] = Xs[r][c];
+    }
+}
+
 hipError_t TrsmDiag(int kb, int N, const float* T, int ldt, bool fwd, bool unit, bool transT, float* B, int ldb,
                     hipStream_t stream)
 {
@@ -517,7 +717,13 @@ hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, con
     // The split forms are (near) FP32-accurate; without packed operands they run as FP32.
     if (prec == Precision::FP32 || prec == Precision::BF16X6 || prec == Precision::FP16X3)
     {
-        if (transA)
+        if (!transA && blocks < 512)
+        {
+            const int tm64 = (M + 63) / 64, tn64 = (N + 63) / 64;
+            hipLaunchKernelGGL(gemm_sub_f32_64_kernel, dim3(tm64 * tn64), dim3(256), 0, stream, M, N, K, A, lda, B, ldb,
+                               C, ldc, tm64, tn64);
+        }
+        else if (transA)
             hipLaunchKernelGGL(gemm_sub_f32_kernel<1>, dim3(blocks), dim3(256), 0, stream, M, N, K, A, lda, B, ldb, C,
                                ldc, tilesM, tilesN);
         else
@@ -551,6 +757,24 @@ hipError_t GemmSub(int M, int N, int K, const float* A, int lda, int transA, con
             hipLaunchKernelGGL((gemm_sub_half_kernel<__bf16, 0, false>), dim3(blocks), dim3(256), 0, stream, M, N, K,
                                A, lda, B, ldb, C, ldc, tilesM, tilesN);
     }
+    return hipGetLastError();
+}
+
+size_t TrsmLowerUnitWorkspaceBytes(int K) { return static_cast<size_t>((K + 63) / 64) * 4096 * sizeof(float); }
+
+hipError_t TrsmLowerUnitInverse(int K, const float* T, int ldt, float* linv, hipStream_t stream)
+{
+    if (K <= 0 || K > kTrsmLuMaxK) return K <= 0 ? hipSuccess : hipErrorInvalidValue;
+    hipLaunchKernelGGL(trtri_unit_lower64_kernel, dim3((K + 63) / 64), dim3(256), 0, stream, K, T, ldt, linv);
+    return hipGetLastError();
+}
+
+hipError_t TrsmLowerUnitApply(int K, int N, const float* T, int ldt, const float* linv, float* B, int ldb,
+                              hipStream_t stream)
+{
+    if (K <= 0 || N <= 0) return hipSuccess;
+    if (K > kTrsmLuMaxK) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(trsm_lu_kernel, dim3((N + 31) / 32), dim3(256), 0, stream, K, N, T, ldt, linv, B, ldb);
     return hipGetLastError();
 }
 

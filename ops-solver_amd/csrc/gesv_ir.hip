@@ -359,13 +359,17 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
             residual(w.x, w.r);
             hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.r, w.scal, S_RNORM);
             hipLaunchKernelGGL(reduce_kernel<0>, dim3(1), dim3(1024), 0, stream, n, w.x, w.scal, S_XNORM);
-            int chainErr = 0;
+            int chainErr = 0, leafErr = 0;
+            int* leafErrWord = useRbt ? nullptr : RealGetrfErrorWord(w.getrfWork, n);
             if ((e = hipMemcpyAsync(hscal.data(), w.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
                     hipSuccess ||
                 (e = hipMemcpyAsync(&chainErr, ts.err, sizeof(int), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+                (leafErrWord != nullptr &&
+                 (e = hipMemcpyAsync(&leafErr, leafErrWord, sizeof(int), hipMemcpyDeviceToHost, stream)) !=
+                     hipSuccess) ||
                 (e = hipStreamSynchronize(stream)) != hipSuccess)
                 return e;
-            if (chainErr != 0) return hipErrorLaunchTimeOut;
+            if (chainErr != 0 || leafErr != 0) return hipErrorLaunchTimeOut;
             const double berr = hscal[S_RNORM] / (hscal[S_ANORM] * hscal[S_XNORM] + hscal[S_BNORM]);
             if (trace)
             {

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstdlib>
 #include <type_traits>
 
 #include "lu_kernels.h"
@@ -39,12 +40,25 @@ constexpr int kMaxRows = 16384;  // largest m the register-resident leaf support
 // one is still being applied); then the global permutation (m ints).
 constexpr int NBMAX = 512;
 constexpr int kMovedInts = 4 * NBMAX + 8;
+// After the ints (8-byte aligned): the exchange slots of the multi-workgroup leaf (two
+// column parities x kMaxLeafGroups workgroups x kSlotWords tagged 64-bit words) and its
+// timeout flag.
+constexpr int kSlotWords = 128;     // per workgroup: 2 key words + up to 64 row words
+constexpr int kMaxLeafGroups = 64;  // workgroups of one multi-workgroup leaf
+constexpr size_t kSlotBytes = 2ull * kMaxLeafGroups * kSlotWords * sizeof(unsigned long long);
+constexpr size_t kLinvPanelBytes = (NBMAX / 64) * 4096 * sizeof(float);  // block inverses of one panel's L11
 struct Work
 {
     int* rowsrc;
     int* moved[2];
     int* perm;
+    unsigned long long* slots;
+    int* err;
+    unsigned* epoch;       // host counter: tag base of the next multi-workgroup leaf
+    float* linvNode;       // block inverses for the panel-internal solves (panel stream only)
+    float* linvPanel[2];   // block inverses of panel k's L11 in slot k % 2 (both streams read them)
 };
+inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
 inline Work Layout(void* base, int m)
 {
     int* p = static_cast<int*>(base);
@@ -53,15 +67,25 @@ inline Work Layout(void* base, int m)
     w.moved[0] = p + m;
     w.moved[1] = w.moved[0] + kMovedInts;
     w.perm = w.moved[1] + kMovedInts;
+    uintptr_t q = reinterpret_cast<uintptr_t>(p + WorkInts(m));
+    q = (q + 7) & ~static_cast<uintptr_t>(7);
+    w.slots = reinterpret_cast<unsigned long long*>(q);
+    w.err = reinterpret_cast<int*>(q + kSlotBytes);
+    w.epoch = nullptr;
+    q = (q + kSlotBytes + 16 + 255) & ~static_cast<uintptr_t>(255);
+    w.linvNode = reinterpret_cast<float*>(q);
+    w.linvPanel[0] = reinterpret_cast<float*>(q + kLinvPanelBytes);
+    w.linvPanel[1] = reinterpret_cast<float*>(q + 2 * kLinvPanelBytes);
     return w;
 }
-inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
 
 template <int CTRL>
 __device__ inline unsigned long long DppU64(unsigned long long v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xffffffffull), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, 0xF, 0xF, false);
+    // Every control used here reads a valid lane, so the old value (v itself) is never kept.
+    const int vlo = static_cast<int>(v & 0xffffffffull), vhi = static_cast<int>(v >> 32);
+    const int lo = __builtin_amdgcn_update_dpp(vlo, vlo, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(vhi, vhi, CTRL, 0xF, 0xF, false);
     return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
 }
 
@@ -284,6 +308,299 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Tall leaf over several workgroups: rows [c0, m) are split over P = gridDim.x one-wave
+// workgroups of 64*R consecutive rows, so each CU does 1/P of the per-column work and no
+// workgroup barrier sits on the per-column path. Per column every wave reduces its
+// candidates to one 64-bit key (DPP) and publishes the key, then its candidate's W values,
+// as tagged 64-bit words (relaxed agent-scope atomic stores; tag = this launch's base +
+// column, so each word is valid on its own and no fence or counter is needed); then it
+// polls the P keys, takes the maximum and reads the winning row. The update of columns
+// beyond j+1 by pivot j is deferred to the next column, where it runs between publishing
+// the key and publishing the row, i.e. while the peers' keys are in flight; per element the
+// updates still come in pivot order, so the result equals the unblocked elimination.
+// Slots alternate with the column parity: a workgroup publishes column j+2 only after all
+// have published j+1, i.e. after all have finished reading column j. Pivot choice, scaling
+// and update order are those of leaf_getf2_kernel (a NaN candidate ranks below every
+// number; if all candidates are NaN the row at position j is the pivot). Spins are bounded:
+// on a timeout the workgroup raises *err, fills its rows with NaN and leaves the column
+// loop; its peers then time out in turn.
+// ---------------------------------------------------------------------------------------
+constexpr unsigned kLeafSpin = 1u << 20;
+
+__device__ inline unsigned long long LoadSlot(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void StoreSlot(unsigned long long* p, unsigned tag, unsigned payload)
+{
+    __hip_atomic_store(p, (static_cast<unsigned long long>(tag) << 32) | payload, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int R, int W>
+__global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
+                                                        int panelEnd, int pivoting, int newPanel,
+                                                        int* __restrict__ ipiv, int* __restrict__ info,
+                                                        int* __restrict__ rowsrc, unsigned long long* slots,
+                                                        unsigned tagBase, int* err)
+{
+    constexpr int T = 64;
+    static_assert(W + 2 <= kSlotWords && 2 * W <= T, "slot layout");
+    constexpr int MOVES = 2 * W;
+    __shared__ float s_row[W];
+    __shared__ float s_prow[2][W];  // pivot rows of columns j (slot j % 2) and j - 1
+    __shared__ int s_bpos[W];
+    __shared__ int s_count;
+    __shared__ int s_mdst[MOVES], s_msrc[MOVES];
+
+    const int lane = threadIdx.x;
+    const int P = gridDim.x, g = blockIdx.x;
+    const int mp = m - c0;
+    float* Ap = A + static_cast<size_t>(c0) * lda + c0;
+    const int rbase = g * T * R;
+    // An earlier leaf of this factorization timed out: its factors are NaN already.
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+
+    float a[R][W];
+    int pos[R], srcv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int lr = rbase + lane + i * T;
+        const bool valid = lr < mp;
+#pragma unroll
+        for (int j = 0; j < W; ++j) a[i][j] = (valid && j < wact) ? Ap[static_cast<size_t>(j) * lda + lr] : 0.0f;
+        pos[i] = valid ? lr : -1;
+        // Panel-level sources, read before any workgroup can store (stores follow the
+        // last column's exchange, which every workgroup enters after these loads).
+        srcv[i] = valid ? (newPanel ? c0 + lr : rowsrc[c0 + lr]) : 0;
+    }
+    if (lane == 0) s_count = 0;
+    int zeroCol = 0;
+    bool aborted = false;  // uniform (one wave)
+    float lprev[R];  // multipliers of the previous pivot (0 for rows not below it)
+#pragma unroll
+    for (int i = 0; i < R; ++i) lprev[i] = 0.0f;
+
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+    {
+        if (j < wact && !aborted)  // uniform
+        {
+            const unsigned tag = tagBase + static_cast<unsigned>(j);
+            unsigned long long* base = slots + static_cast<size_t>(j & 1) * kMaxLeafGroups * kSlotWords;
+            unsigned long long* mine = base + g * kSlotWords;
+            // Column j is up to date; columns beyond it still lack the previous pivot.
+            unsigned long long best = 0ull;
+            int bi = -1;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
+                const unsigned ab = __float_as_uint(fabsf(a[i][j]));
+                const unsigned long long k =
+                    cand ? (1ull << 63) |
+                               (static_cast<unsigned long long>(ab <= 0x7f800000u ? ab + 1u : 0u) << 32) |
+                               static_cast<unsigned>(~pos[i])
+                         : 0ull;
+                bi = k > best ? i : bi;
+                best = k > best ? k : best;
+            }
+            const unsigned long long top = WaveMaxU64(best);
+            if (lane == 0)
+            {
+                StoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
+                StoreSlot(mine + 1, tag, static_cast<unsigned>(top));
+            }
+            // Deferred: the previous pivot on columns j+1 .. W-1.
+            if (j > 0)
+#pragma unroll
+                for (int k = j + 1; k < W; ++k)
+                {
+                    const float pk = s_prow[(j - 1) & 1][k];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) a[i][k] = fmaf(-lprev[i], pk, a[i][k]);
+                }
+            // The candidate's row, now complete, through LDS to lanes 0 .. wact-1.
+            if (best == top && bi >= 0)
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (i == bi)
+#pragma unroll
+                        for (int k = 0; k < W; ++k) s_row[k] = a[i][k];
+            __syncthreads();
+            if (lane < wact) StoreSlot(mine + 2 + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
+            // The P keys (lane l reads workgroup l's two key words).
+            unsigned long long key = 0ull;
+            bool ok = false;
+            for (unsigned spins = 0; spins < kLeafSpin; ++spins)
+            {
+                unsigned long long hi = 0ull, lo = 0ull;
+                if (lane < P)
+                {
+                    hi = LoadSlot(base + lane * kSlotWords);
+                    lo = LoadSlot(base + lane * kSlotWords + 1);
+                }
+                const bool good =
+                    lane >= P || (static_cast<unsigned>(hi >> 32) == tag && static_cast<unsigned>(lo >> 32) == tag);
+                key = (hi << 32) | (lo & 0xffffffffull);
+                if (__all(good))
+                {
+                    ok = true;
+                    break;
+                }
+            }
+            const unsigned long long gtop = WaveMaxU64(lane < P ? key : 0ull);
+            const unsigned long long hit = __ballot(ok && lane < P && key == gtop);
+            const int winner = hit != 0ull ? __ffsll(static_cast<long long>(hit)) - 1 : 0;
+            // The winner's row: lane k holds its value in column k.
+            unsigned long long rw = 0ull;
+            for (unsigned spins = 0; ok; ++spins)
+            {
+                rw = lane < wact ? LoadSlot(base + winner * kSlotWords + 2 + lane)
+                                 : static_cast<unsigned long long>(tag) << 32;
+                if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
+                if (spins >= kLeafSpin) ok = false;
+            }
+            if (!ok)
+            {
+                if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                aborted = true;
+                continue;
+            }
+            if (lane < W) s_prow[j & 1][lane] = __uint_as_float(static_cast<unsigned>(rw));
+            __syncthreads();
+            const int bpos = gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                if (pos[i] == bpos)
+                    pos[i] = j;
+                else if (pos[i] == j)
+                    pos[i] = bpos;
+            }
+            if (lane == 0)
+            {
+                s_bpos[j] = bpos;
+                if (g == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
+            }
+            const float d = s_prow[j & 1][j];
+            if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
+            const bool recip = fabsf(d) >= FLT_MIN;  // uniform
+            const float r = 1.0f / d;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const bool below = pos[i] > j && d != 0.0f;
+                float q = a[i][j] * r;
+                if (!recip) q = a[i][j] / d;  // uniform branch
+                lprev[i] = below ? q : 0.0f;
+                a[i][j] = below ? q : a[i][j];
+            }
+            // Column j+1 now (the next candidate search needs it); the rest next column.
+            if (j + 1 < W)
+            {
+                const float pk = s_prow[j & 1][j + 1];
+#pragma unroll
+                for (int i = 0; i < R; ++i) a[i][j + 1] = fmaf(-lprev[i], pk, a[i][j + 1]);
+            }
+        }
+    }
+    if (aborted)
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < W; ++j) a[i][j] = __builtin_nanf("");
+
+    // Each row to its final position (leaf columns), and its panel-level source.
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int lr = rbase + lane + i * T;
+        if (lr < mp)
+        {
+            const int dst = pos[i];
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < wact) Ap[static_cast<size_t>(j) * lda + dst] = a[i][j];
+            rowsrc[c0 + dst] = srcv[i];
+        }
+    }
+    if (g == 0 && lane == 0 && zeroCol != 0 && *info == 0) *info = zeroCol;
+    if (aborted) return;  // uniform
+    __syncthreads();
+
+    // The moved rows, from the interchange sequence (identical in every workgroup): the row
+    // that ends at position p came from the position found by undoing the interchanges in
+    // reverse. Candidates: positions 0..wact-1 (lanes 0..W-1) and the pivot positions
+    // beyond them (lanes W..2W-1, first occurrence only).
+    {
+        int p = -1;
+        if (lane < W)
+            p = lane < wact ? lane : -1;
+        else if (lane - W < wact)
+        {
+            const int jj = lane - W;
+            p = s_bpos[jj] >= wact ? s_bpos[jj] : -1;
+            for (int t = 0; t < jj && p >= 0; ++t)
+                if (s_bpos[t] == p) p = -1;
+        }
+        if (p >= 0)
+        {
+            int x = p;
+            for (int jj = wact - 1; jj >= 0; --jj)
+            {
+                const int b = s_bpos[jj];
+                x = x == jj ? b : (x == b ? jj : x);
+            }
+            if (x != p)
+            {
+                const int slot = atomicAdd(&s_count, 1);
+                s_mdst[slot] = c0 + p;
+                s_msrc[slot] = c0 + x;
+            }
+        }
+    }
+    __syncthreads();
+
+    // The same interchanges on the other panel columns [j0, c0) and [c0 + wact, panelEnd),
+    // columns dealt round-robin to the workgroups; each column is gathered completely
+    // before it is scattered.
+    const int cnt = s_count;
+    const int left = c0 - j0, others = left + (panelEnd - c0 - wact);
+    constexpr int SWB = 4;
+    for (int q0 = g; q0 < others; q0 += SWB * P)  // uniform trip count
+    {
+        float v[SWB];
+        int cc[SWB];
+#pragma unroll
+        for (int bb = 0; bb < SWB; ++bb)
+        {
+            const int q = q0 + bb * P;
+            cc[bb] = -1;
+            v[bb] = 0.0f;
+            if (q < others && lane < cnt)
+            {
+                cc[bb] = q < left ? j0 + q : c0 + wact + (q - left);
+                v[bb] = A[static_cast<size_t>(cc[bb]) * lda + s_msrc[lane]];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < SWB; ++bb)
+            if (cc[bb] >= 0) A[static_cast<size_t>(cc[bb]) * lda + s_mdst[lane]] = v[bb];
+    }
+}
+
+__global__ void clear_slots_kernel(unsigned long long* slots, int words, int* err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) slots[i] = 0ull;
+    if (i == 0) *err = 0;
+}
+
 // Builds the list of rows moved by the whole panel (rowsrc[r] != r) and applies it to the
 // global permutation. One workgroup.
 __global__ __launch_bounds__(1024) void compact_moves_kernel(int m, int j0, const int* __restrict__ rowsrc,
@@ -373,10 +690,38 @@ hipError_t LaunchLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pa
     return hipGetLastError();
 }
 
+// Panels with more active rows than this run their leaves over several workgroups
+// (leaf_multi_kernel); OPS_SOLVER_MULTI_LEAF_ROWS overrides it (0: never).
+int MultiLeafRows()
+{
+    static const int rows = []
+    {
+        const char* e = std::getenv("OPS_SOLVER_MULTI_LEAF_ROWS");
+        return e != nullptr ? std::atoi(e) : 2048;
+    }();
+    return rows;
+}
+inline bool UseMultiLeaf(int mp) { return MultiLeafRows() > 0 && mp > MultiLeafRows(); }
+
+constexpr int kMultiT = 64, kMultiR = 4, kMultiW = 32;  // one wave of 256 rows per workgroup, 32 columns
+
+hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
+                     int* ipiv, int* info, const Work& w, hipStream_t stream)
+{
+    const int mp = m - c0;
+    const int groups = (mp + kMultiT * kMultiR - 1) / (kMultiT * kMultiR);
+    if (groups > kMaxLeafGroups || wact > kMultiW || w.epoch == nullptr) return hipErrorInvalidValue;
+    const unsigned tagBase = (*w.epoch)++ * 64u;
+    hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW>), dim3(groups), dim3(kMultiT), 0, stream, A, lda,
+                       m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w.rowsrc, w.slots, tagBase, w.err);
+    return hipGetLastError();
+}
+
 // Leaf width for a panel of mp active rows: every shape keeps R*W <= 128 values per lane
 // (512-thread workgroups leave up to 256 VGPRs per lane).
 int LeafWidth(int mp)
 {
+    if (UseMultiLeaf(mp)) return kMultiW;
     if (mp <= 512) return 32;
     if (mp <= 1024) return 16;
     if (mp <= 8192) return 8;
@@ -387,6 +732,7 @@ hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd
                 int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
     const int mp = m - c0;
+    if (UseMultiLeaf(mp)) return MultiLeaf(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream);
 #define LEAF(T, R, W) \
     LaunchLeaf<T, R, W>(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream)
     if (mp <= 256) return LEAF(256, 1, 32);
@@ -419,8 +765,9 @@ hipError_t PanelRec(float* a, int lda, int m, int j0, int panelEnd, int c0, int 
     const int h = (w / 2 + W - 1) / W * W;  // whole leaves on the left
     RET_IF(PanelRec(a, lda, m, j0, panelEnd, c0, h, pivoting, ipiv, info, wk, stream));
     const int r = w - h;
-    RET_IF(TrsmBlock(h, r, a + static_cast<size_t>(c0) * lda + c0, lda, true, true, false,
-                     a + static_cast<size_t>(c0 + h) * lda + c0, lda, stream));
+    const float* L11 = a + static_cast<size_t>(c0) * lda + c0;
+    RET_IF(TrsmLowerUnitInverse(h, L11, lda, wk.linvNode, stream));
+    RET_IF(TrsmLowerUnitApply(h, r, L11, lda, wk.linvNode, a + static_cast<size_t>(c0 + h) * lda + c0, lda, stream));
     if (m - c0 - h > 0)
         RET_IF(GemmSub(m - c0 - h, r, h, a + static_cast<size_t>(c0) * lda + c0 + h, lda, 0,
                        a + static_cast<size_t>(c0 + h) * lda + c0, lda, a + static_cast<size_t>(c0 + h) * lda + c0 + h,
@@ -440,8 +787,10 @@ int OuterBlock(int m, int n, Precision prec)
 size_t RealGetrfWorkspaceBytes(int m, int n)
 {
     (void)n;
-    return WorkInts(std::max(m, 1)) * sizeof(int);
+    return WorkInts(std::max(m, 1)) * sizeof(int) + 8 + kSlotBytes + 16 + 256 + 3 * kLinvPanelBytes;
 }
+
+int* RealGetrfErrorWord(void* work, int m) { return UseMultiLeaf(m) ? Layout(work, std::max(m, 1)).err : nullptr; }
 
 // ipiv == nullptr selects no pivoting. perm (optional, m ints) receives the row
 // permutation P with (P A)[i] = A[perm[i]]. pack (optional, BF16 / BF16X3 trailing
@@ -452,7 +801,14 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     if (m > kMaxRows) return hipErrorInvalidValue;
     Work w = Layout(work, m);
     if (permOut) w.perm = permOut;
+    unsigned epoch = 1;
+    w.epoch = &epoch;
     FillInfo(info, 1, 0, stream);
+    if (UseMultiLeaf(m))
+    {
+        constexpr int words = static_cast<int>(kSlotBytes / sizeof(unsigned long long));
+        hipLaunchKernelGGL(clear_slots_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, w.slots, words, w.err);
+    }
     const int kmax = std::min(m, n);
     if (kmax == 0) return hipGetLastError();
     const int pivoting = ipiv != nullptr;
@@ -478,7 +834,9 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         if (pivoting)
             hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, s0, m, j0, w.rowsrc, w.moved[k & 1],
                                permOut);
-        return hipGetLastError();
+        RET_IF(hipGetLastError());
+        // The U12 solves of both streams use the block inverses of this panel's L11.
+        return TrsmLowerUnitInverse(nb, a + static_cast<size_t>(j0) * lda + j0, lda, w.linvPanel[k & 1], s0);
     };
     // Applies panel k to the columns [lo, hi) (interchanges, U12 solve, trailing product)
     // and, when leftSwaps, its interchanges to the columns left of the panel.
@@ -491,8 +849,8 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
                                hi, w.moved[k & 1]);
         RET_IF(hipGetLastError());
         if (cols <= 0) return hipSuccess;
-        RET_IF(TrsmBlock(nb, cols, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, false,
-                         a + static_cast<size_t>(lo) * lda + j0, lda, st));
+        RET_IF(TrsmLowerUnitApply(nb, cols, a + static_cast<size_t>(j0) * lda + j0, lda, w.linvPanel[k & 1],
+                                  a + static_cast<size_t>(lo) * lda + j0, lda, st));
         if (m - pe <= 0) return hipSuccess;
         const float* L21 = a + static_cast<size_t>(j0) * lda + pe;
         const float* U12 = a + static_cast<size_t>(lo) * lda + j0;

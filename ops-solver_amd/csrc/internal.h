@@ -110,6 +110,10 @@ enum class Precision
 };
 // Workspace bytes needed by RealGetrf for an m x n matrix.
 size_t RealGetrfWorkspaceBytes(int m, int n);
+// Device address of the factorization's timeout flag inside its workspace (nonzero after a
+// multi-workgroup leaf gave up waiting for its peers; the factors then hold NaN), or
+// nullptr when a factorization of m rows runs no such leaf.
+int* RealGetrfErrorWord(void* work, int m);
 // Blocked right-looking LU with partial pivoting (or none when ipiv == nullptr).
 // Writes ipiv (1-based) and info (first zero pivot). Returns hipSuccess on launch success.
 hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,

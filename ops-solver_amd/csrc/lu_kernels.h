@@ -28,4 +28,13 @@ hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const flo
 hipError_t TrsmBlock(int K, int N, const float* T, int ldt, bool lower, bool unit, bool transT, float* B, int ldb,
                      hipStream_t stream);
 
+// B <- T^{-1} B for a unit-lower K x K block T (K <= 512), in two steps: Inverse writes
+// the inverses of T's 64 x 64 diagonal blocks to linv (TrsmLowerUnitWorkspaceBytes), Apply
+// then solves any number of column blocks in one launch (one workgroup per 32 columns,
+// products on FP32 MFMA). One Inverse serves every Apply with the same T.
+size_t TrsmLowerUnitWorkspaceBytes(int K);
+hipError_t TrsmLowerUnitInverse(int K, const float* T, int ldt, float* linv, hipStream_t stream);
+hipError_t TrsmLowerUnitApply(int K, int N, const float* T, int ldt, const float* linv, float* B, int ldb,
+                              hipStream_t stream);
+
 }  // namespace ops

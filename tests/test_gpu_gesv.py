@@ -37,7 +37,7 @@ def _system(torch, n, cond, seed):
 
 
 @pytest.mark.parametrize("name", ["SB", "SH", "SS"])
-@pytest.mark.parametrize("n,cond", [(1024, 1e4), (2048, 1e6), (1100, 1e4)])
+@pytest.mark.parametrize("n,cond", [(1024, 1e4), (2048, 1e6), (1100, 1e4), (4700, 1e5)])
 def test_gesv_refined(torch, name, n, cond):
     a, at, b, x_true = _system(torch, n, cond, seed=n)
     with A.Handle() as h:

@@ -57,7 +57,8 @@ def _check_pivots(piv, ref_piv, margin):
 
 
 @pytest.mark.parametrize("m,n,seed", [(512, 512, 1), (64, 64, 2), (300, 200, 3), (200, 300, 4), (1, 1, 5),
-                                      (5, 7, 6), (1000, 1000, 7), (2100, 2100, 8), (33, 1, 9), (4100, 256, 10)])
+                                      (5, 7, 6), (1000, 1000, 7), (2100, 2100, 8), (33, 1, 9), (4100, 256, 10),
+                                      (8200, 300, 11), (16384, 72, 12)])
 def test_sgetrf_parity(torch, handle, m, n, seed):
     rng = np.random.default_rng(seed)
     a = (rng.standard_normal((m, n)) / np.sqrt(n)).astype(np.float32)
@@ -165,10 +166,34 @@ def test_singular_and_zero_columns(torch, handle):
     assert st == A.SUCCESS and info == ref_info == 41
 
 
+def test_tall_zero_column_and_ties(torch, handle):
+    """Tall panels (leaves spread over several workgroups): an exactly zero column gives
+    info at that column and the elimination skips it; equal-magnitude candidates in
+    different row blocks resolve to the lowest row, as LAPACK's first-maximum rule."""
+    m, n = 9000, 40
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((m, n)).astype(np.float32)
+    a[:, 17] = 0.0
+    # column 0: the same largest magnitude at rows 7000, 600 and 8999 -> row 600 wins
+    a[:, 0] = np.clip(a[:, 0], -3.0, 3.0)
+    a[7000, 0], a[600, 0], a[8999, 0] = -8.0, 8.0, -8.0
+    st, lu, piv, info, _ = _sgetrf(torch, handle, a)
+    ref_lu, ref_piv, ref_info, margin = O.sgetrf(a, margins=True)
+    assert st == A.SUCCESS and info == ref_info == 18
+    assert piv[0] == ref_piv[0] == 601
+    _check_pivots(piv[:17], ref_piv[:17], np.minimum(margin[:17], 1.0) + (np.arange(17) == 0))
+
+
 def test_determinism(torch, handle):
     n = 700
     rng = np.random.default_rng(99)
     a = rng.standard_normal((n, n)).astype(np.float32)
+    outs = [_sgetrf(torch, handle, a)[1:4] for _ in range(3)]
+    for lu, piv, info in outs[1:]:
+        assert np.array_equal(lu.view(np.uint32), outs[0][0].view(np.uint32))
+        assert np.array_equal(piv, outs[0][1]) and info == outs[0][2]
+    # a tall panel, whose leaves exchange pivots between workgroups
+    a = rng.standard_normal((6000, 200)).astype(np.float32)
     outs = [_sgetrf(torch, handle, a)[1:4] for _ in range(3)]
     for lu, piv, info in outs[1:]:
         assert np.array_equal(lu.view(np.uint32), outs[0][0].view(np.uint32))
