@@ -44,7 +44,7 @@ constexpr int kMovedInts = 4 * NBMAX + 8;
 // column parities x kMaxLeafGroups workgroups x kSlotWords tagged 64-bit words) and its
 // timeout flag.
 constexpr int kSlotWords = 128;     // per workgroup: 2 key words + up to 64 row words
-constexpr int kMaxLeafGroups = 64;  // workgroups of one multi-workgroup leaf
+constexpr int kMaxLeafGroups = 128;  // workgroups of one multi-workgroup leaf
 constexpr size_t kSlotBytes = 2ull * kMaxLeafGroups * kSlotWords * sizeof(unsigned long long);
 constexpr size_t kLinvPanelBytes = (NBMAX / 64) * 4096 * sizeof(float);  // block inverses of one panel's L11
 struct Work
@@ -432,29 +432,40 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                         for (int k = 0; k < W; ++k) s_row[k] = a[i][k];
             __syncthreads();
             if (lane < wact) StoreSlot(mine + 2 + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
-            // The P keys (lane l reads workgroup l's two key words).
-            unsigned long long key = 0ull;
+            // The P keys: lane l reads the two key words of workgroups l and l + 64.
+            unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
             for (unsigned spins = 0; spins < kLeafSpin; ++spins)
             {
-                unsigned long long hi = 0ull, lo = 0ull;
+                unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
                 {
-                    hi = LoadSlot(base + lane * kSlotWords);
-                    lo = LoadSlot(base + lane * kSlotWords + 1);
+                    hi0 = LoadSlot(base + lane * kSlotWords);
+                    lo0 = LoadSlot(base + lane * kSlotWords + 1);
                 }
-                const bool good =
-                    lane >= P || (static_cast<unsigned>(hi >> 32) == tag && static_cast<unsigned>(lo >> 32) == tag);
-                key = (hi << 32) | (lo & 0xffffffffull);
-                if (__all(good))
+                if (lane + 64 < P)
+                {
+                    hi1 = LoadSlot(base + (lane + 64) * kSlotWords);
+                    lo1 = LoadSlot(base + (lane + 64) * kSlotWords + 1);
+                }
+                const bool good0 =
+                    lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
+                const bool good1 = lane + 64 >= P ||
+                                   (static_cast<unsigned>(hi1 >> 32) == tag && static_cast<unsigned>(lo1 >> 32) == tag);
+                key0 = lane < P ? (hi0 << 32) | (lo0 & 0xffffffffull) : 0ull;
+                key1 = lane + 64 < P ? (hi1 << 32) | (lo1 & 0xffffffffull) : 0ull;
+                if (__all(good0 && good1))
                 {
                     ok = true;
                     break;
                 }
             }
-            const unsigned long long gtop = WaveMaxU64(lane < P ? key : 0ull);
-            const unsigned long long hit = __ballot(ok && lane < P && key == gtop);
-            const int winner = hit != 0ull ? __ffsll(static_cast<long long>(hit)) - 1 : 0;
+            const unsigned long long gtop = WaveMaxU64(key0 > key1 ? key0 : key1);
+            const unsigned long long hit0 = __ballot(ok && lane < P && key0 == gtop);
+            const unsigned long long hit1 = __ballot(ok && lane + 64 < P && key1 == gtop);
+            const int winner = hit0 != 0ull   ? __ffsll(static_cast<long long>(hit0)) - 1
+                               : hit1 != 0ull ? 64 + __ffsll(static_cast<long long>(hit1)) - 1
+                                              : 0;
             // The winner's row: lane k holds its value in column k.
             unsigned long long rw = 0ull;
             for (unsigned spins = 0; ok; ++spins)
@@ -703,7 +714,7 @@ int MultiLeafRows()
 }
 inline bool UseMultiLeaf(int mp) { return MultiLeafRows() > 0 && mp > MultiLeafRows(); }
 
-constexpr int kMultiT = 64, kMultiR = 4, kMultiW = 32;  // one wave of 256 rows per workgroup, 32 columns
+constexpr int kMultiT = 64, kMultiR = 2, kMultiW = 32;  // one wave of 128 rows per workgroup, 32 columns
 
 hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
                      int* ipiv, int* info, const Work& w, hipStream_t stream)
@@ -766,8 +777,14 @@ hipError_t PanelRec(float* a, int lda, int m, int j0, int panelEnd, int c0, int 
     RET_IF(PanelRec(a, lda, m, j0, panelEnd, c0, h, pivoting, ipiv, info, wk, stream));
     const int r = w - h;
     const float* L11 = a + static_cast<size_t>(c0) * lda + c0;
-    RET_IF(TrsmLowerUnitInverse(h, L11, lda, wk.linvNode, stream));
-    RET_IF(TrsmLowerUnitApply(h, r, L11, lda, wk.linvNode, a + static_cast<size_t>(c0 + h) * lda + c0, lda, stream));
+    float* B12 = a + static_cast<size_t>(c0 + h) * lda + c0;
+    if (h <= 64)  // one diagonal block: a single substitution launch
+        RET_IF(TrsmBlock(h, r, L11, lda, true, true, false, B12, lda, stream));
+    else
+    {
+        RET_IF(TrsmLowerUnitInverse(h, L11, lda, wk.linvNode, stream));
+        RET_IF(TrsmLowerUnitApply(h, r, L11, lda, wk.linvNode, B12, lda, stream));
+    }
     if (m - c0 - h > 0)
         RET_IF(GemmSub(m - c0 - h, r, h, a + static_cast<size_t>(c0) * lda + c0 + h, lda, 0,
                        a + static_cast<size_t>(c0 + h) * lda + c0, lda, a + static_cast<size_t>(c0 + h) * lda + c0 + h,
