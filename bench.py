@@ -316,7 +316,7 @@ def main():
     flops = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
     sb = res["SB"]
     value = flops / (sb["ms"] * 1e-3) / 1e9
-    launches, gflop, gms = sb["profile"]
+    launches, gflop, gms = sb["profile"] if sb["profile"] is not None else (0, 0.0, 0.0)
     achieved = gflop / (gms * 1e-3) if gms > 0 else 0.0
     out = {
         "metric": METRIC,
