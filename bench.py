@@ -192,6 +192,84 @@ def batched_inverse(A, torch, h, steps=20):
             "max_abs_AC_minus_I_sample": res, "all_info_zero": bool((info == 0).all().item())}
 
 
+def complex_solve(A, torch, h, n=8192, nrhs=16, steps=2):
+    """cgetrf + cgetrs (device column-major) of one complex64 system of order n with nrhs
+    right-hand sides, re/im i.i.d. N(0, 1/2)/sqrt(n) plus 1.5 on the diagonal; and
+    cmatinv_batched on 4096 complex64 matrices of 64x64 with the same distribution."""
+    L = A.lib()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    a0 = torch.complex(torch.randn(n, n, device="cuda", generator=g), torch.randn(n, n, device="cuda", generator=g))
+    a0 = a0 * (0.5 ** 0.5 / n ** 0.5) + 1.5 * torch.eye(n, device="cuda", dtype=torch.complex64)
+    b0 = torch.complex(torch.randn(nrhs, n, device="cuda", generator=g), torch.randn(nrhs, n, device="cuda", generator=g))
+    a = torch.empty_like(a0)  # rows of the row-major tensor = columns of the column-major matrix
+    b = torch.empty_like(b0)
+    lw = ctypes.c_int(0)
+    assert L.aclsolverCgetrf_bufferSize(h.h, n, n, A.ptr(a), n, ctypes.byref(lw)) == A.SUCCESS
+    work = torch.empty(max(lw.value, 1), dtype=torch.complex64, device="cuda")
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    info = torch.empty(1, dtype=torch.int32, device="cuda")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for s in range(steps + 1):
+        a.copy_(a0)
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        ev0.record()
+        assert L.aclsolverCgetrf(h.h, n, n, A.ptr(a), n, A.ptr(work), A.ptr(ipiv), A.ptr(info)) == A.SUCCESS
+        assert L.aclsolverCgetrsDevice(h.h, 0, n, nrhs, A.ptr(a), n, A.ptr(ipiv), A.ptr(b), n, A.ptr(info)) == A.SUCCESS
+        ev1.record()
+        torch.cuda.synchronize()
+        if s:
+            times.append(ev0.elapsed_time(ev1))
+        log(f"complex n={n} step {s}: {ev0.elapsed_time(ev1):.2f} ms")
+    ms = sum(times) / len(times)
+    # backward error in complex128 (column-major A: matrix = a0^T)
+    a64 = a0.T.to(torch.complex128)
+    x64 = b.T.to(torch.complex128)
+    r = b0.T.to(torch.complex128) - a64 @ x64
+    berr = (r.abs().max() / (a64.abs().sum(1).max() * x64.abs().max() + b0.abs().max())).item()
+    flops = 8.0 / 3.0 * n ** 3 + 8.0 * n * n * nrhs
+    out = {"workload": f"cgetrf+cgetrs complex64 n={n}, {nrhs} RHS, re/im N(0,1/2)/sqrt(n) + 1.5 I",
+           "ms": round(ms, 3), "GFLOP_per_s": round(flops / ms / 1e6, 1), "backward_error": berr,
+           "info": int(info.item())}
+    del a, a0, b, b0, work
+    torch.cuda.empty_cache()
+    # cmatinv_batched 4096 x 64x64
+    nb, batch = 64, 4096
+    m0 = torch.complex(torch.randn(batch, nb, nb, device="cuda", generator=g),
+                       torch.randn(batch, nb, nb, device="cuda", generator=g))
+    m0 = (m0 * (0.5 ** 0.5 / nb ** 0.5) + 1.5 * torch.eye(nb, device="cuda", dtype=torch.complex64)).contiguous()
+    mi = torch.empty_like(m0)
+    pa = (m0.data_ptr() + torch.arange(batch, device="cuda", dtype=torch.int64) * nb * nb * 8)
+    pc = (mi.data_ptr() + torch.arange(batch, device="cuda", dtype=torch.int64) * nb * nb * 8)
+    binfo = torch.empty(batch, dtype=torch.int32, device="cuda")
+
+    def inv():
+        assert L.aclsolverCmatinvBatched(h.h, nb, A.ptr(pa), nb, A.ptr(pc), nb, A.ptr(binfo), batch) == A.SUCCESS
+
+    inv()
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(5):
+        inv()
+    ev1.record()
+    torch.cuda.synchronize()
+    ims = ev0.elapsed_time(ev1) / 5
+    idx = torch.arange(0, batch, 511, device="cuda")
+    am = m0[idx].transpose(1, 2).to(torch.complex128)
+    cm = mi[idx].transpose(1, 2).to(torch.complex128)
+    res = (torch.matmul(am, cm) - torch.eye(nb, device="cuda", dtype=torch.complex128)).abs().max().item()
+    bytes_ = batch * (2 * nb * nb * 8 + 4)
+    out["cmatinv_batched"] = {"workload": "4096 complex64 matrices of 64x64, same distribution (inverse + info)",
+                              "ms": round(ims, 4), "matrices_per_s": round(batch / ims * 1e3, 1),
+                              "GB_per_s": round(bytes_ / ims / 1e6, 1),
+                              "hbm_frac": round(bytes_ / ims / 1e-3 / PEAK_HBM, 4),
+                              "GFLOP_per_s": round(8.0 * nb ** 3 * batch / ims / 1e6, 1),
+                              "max_abs_AC_minus_I_sample": res, "all_info_zero": bool((binfo == 0).all().item())}
+    return out
+
+
 def uneven_batched(A, torch, h, steps=5):
     """20000 FP32 systems, n_i ~ U{8..256} (seeded), A_i = N(0,1)/sqrt(n_i) + 2I, 4 RHS:
     getrf + getrs of every system in one persistent launch (plan built and autotuned once,
@@ -363,6 +441,8 @@ def main():
         out["batched_inverse"] = batched_inverse(A, torch, h)
         log("uneven batched getrf+getrs, 20000 systems")
         out["uneven_batched"] = uneven_batched(A, torch, h)
+        log("complex cgetrf+cgetrs n=8192, cmatinv 4096 x 64x64")
+        out["complex"] = complex_solve(A, torch, h)
     if not args.skip_cpu:
         log(f"host LAPACK baseline n={args.cpu_n}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -407,7 +407,7 @@ OPS_API aclsolverStatus_t aclsolverCgetriBatched(aclsolverHandle_t handle, int n
     if (bad) return Invalid(handle, infoArray, batchSize, bad);
     const auto* A2 = reinterpret_cast<const float2* const*>(Aarray);
     auto* C2 = reinterpret_cast<float2* const*>(Carray);
-    if (n <= 32)
+    if (n <= 64)
     {
         if (CGetriSmallBatched(n, A2, lda, PivotArray, C2, ldc, infoArray, batchSize, s) != hipSuccess)
             return ACLSOLVER_STATUS_EXECUTION_FAILED;
@@ -454,7 +454,7 @@ OPS_API aclsolverStatus_t aclsolverCmatinvBatched(aclsolverHandle_t handle, int 
     if (bad) return Invalid(handle, info, batchSize, bad);
     const auto* A2 = reinterpret_cast<const float2* const*>(A);
     auto* C2 = reinterpret_cast<float2* const*>(Ainv);
-    if (n <= 32)
+    if (n <= 64)
     {
         if (CMatinvBatched(n, A2, nullptr, lda, 0, C2, nullptr, lda_inv, 0, info, batchSize, s) != hipSuccess)
             return ACLSOLVER_STATUS_EXECUTION_FAILED;

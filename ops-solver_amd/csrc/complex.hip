@@ -986,7 +986,9 @@ hipError_t CMatinvBatched(int n, const float2* const* Aarr, const float2* Abase,
         return LaunchCMatinv<16>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, info, batch, stream);
     if (n <= 32)
         return LaunchCMatinv<32>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, info, batch, stream);
-    return hipErrorNotSupported;  // 32 < n: the caller takes the factor-and-solve route
+    if (n <= 64)
+        return LaunchCMatinv<64>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, info, batch, stream);
+    return hipErrorNotSupported;  // 64 < n: the caller takes the factor-and-solve route
 }
 
 hipError_t CGetriSmallBatched(int n, const float2* const* Aarr, int lda, const int* piv, float2* const* Carr, int ldc,
@@ -996,6 +998,7 @@ hipError_t CGetriSmallBatched(int n, const float2* const* Aarr, int lda, const i
     if (n <= 8) return LaunchCGetri<8>(n, Aarr, lda, piv, Carr, ldc, info, batch, stream);
     if (n <= 16) return LaunchCGetri<16>(n, Aarr, lda, piv, Carr, ldc, info, batch, stream);
     if (n <= 32) return LaunchCGetri<32>(n, Aarr, lda, piv, Carr, ldc, info, batch, stream);
+    if (n <= 64) return LaunchCGetri<64>(n, Aarr, lda, piv, Carr, ldc, info, batch, stream);
     return hipErrorNotSupported;
 }
 

@@ -150,7 +150,7 @@ def _ptr_array(torch, tensors):
     return torch.tensor([t.data_ptr() for t in tensors], dtype=torch.int64, device="cuda")
 
 
-@pytest.mark.parametrize("n,batch", [(32, 64), (5, 17), (16, 40), (1, 3), (64, 6)])
+@pytest.mark.parametrize("n,batch", [(32, 64), (5, 17), (16, 40), (1, 3), (64, 6), (40, 70)])
 def test_cmatinv_batched(torch, handle, n, batch):
     rng = np.random.default_rng(n * 31 + batch)
     mats = [_crand(rng, n, n, shift=1.5) for _ in range(batch)]
@@ -166,7 +166,28 @@ def test_cmatinv_batched(torch, handle, n, batch):
         assert O.inverse_residual(m, inv) <= 5
 
 
-@pytest.mark.parametrize("n,batch", [(32, 20), (7, 9), (48, 3)])
+def test_cmatinv_batched_singular_member(torch, handle):
+    """One-wave-per-matrix path (33 <= n <= 64): a member with an exactly zero column reports
+    info = that column + 1 (the first zero pivot of the elimination); its neighbours in the
+    same workgroup are unaffected."""
+    n, batch = 50, 9
+    rng = np.random.default_rng(404)
+    mats = [_crand(rng, n, n, shift=1.5) for _ in range(batch)]
+    mats[5][:, 17] = 0
+    dA = [A.to_device_cm(m) for m in mats]
+    dC = [torch.zeros((n, n), dtype=torch.complex64, device="cuda") for _ in range(batch)]
+    info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)
+    st = A.lib().aclsolverCmatinvBatched(handle.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
+    torch.cuda.synchronize()
+    got = info.cpu().numpy()
+    assert st == A.SUCCESS and got[5] == 18 and (np.delete(got, 5) == 0).all()
+    for i, (m, c) in enumerate(zip(mats, dC)):
+        if i != 5:
+            assert O.inverse_residual(m, A.from_device_cm(c, n, n)) <= 5
+
+
+@pytest.mark.parametrize("n,batch", [(32, 20), (7, 9), (48, 3), (64, 11)])
 def test_cgetri_batched_and_single(torch, handle, n, batch):
     rng = np.random.default_rng(n + 5)
     mats = [_crand(rng, n, n, shift=1.5) for _ in range(batch)]
