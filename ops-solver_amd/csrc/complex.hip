@@ -768,6 +768,26 @@ __device__ inline void CGroupArgmax(uint32_t val, int key, uint32_t* bestVal, in
     *bestKey = k;
 }
 
+// Column k of a register row (k uniform but not a compile-time constant): a select chain,
+// so register arrays stay in registers when the loop over k is not unrolled.
+template <int NL>
+__device__ inline float2 GetCol(const float2 (&S)[NL], int k)
+{
+    float2 v = S[0];
+#pragma unroll
+    for (int j = 1; j < NL; ++j) v = (j == k) ? S[j] : v;
+    return v;
+}
+template <int NL>
+__device__ inline void SetCol(float2 (&S)[NL], int k, float2 v)
+{
+#pragma unroll
+    for (int j = 0; j < NL; ++j) S[j] = (j == k) ? v : S[j];
+}
+// Up to 32 columns the loops over k are unrolled (k becomes a constant and the selects
+// fold away); at 64 they stay loops, which keeps those kernels small enough to compile
+// quickly and to fit the instruction cache.
+
 // Gauss-Jordan with implicit pivoting (see batched.hip for the real derivation).
 template <int NL>
 __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* const* __restrict__ Aarr,
@@ -798,12 +818,13 @@ __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* co
     int pos = lane;
     bool used = false;
     int zero = 0;
-#pragma unroll
+#pragma unroll(NL <= 32 ? NL : 1)
     for (int k = 0; k < NL; ++k)
     {
         if (k < n)
         {
-            const uint32_t cand = (rowLive && !used) ? PivotScore(S[k]) : 0u;
+            const float2 sk = GetCol(S, k);
+            const uint32_t cand = (rowLive && !used) ? PivotScore(sk) : 0u;
             uint32_t best;
             int key;
             CGroupArgmax<NL>(rowLive && !used ? cand : 0u, (rowLive && !used) ? ((pos << 8) | lane) : INT_MAX,
@@ -824,13 +845,13 @@ __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* co
             const bool z = d.x == 0.0f && d.y == 0.0f;
             if (z && zero == 0) zero = k + 1;
             const double2 r = Recip64(z ? make_float2(1.0f, 0.0f) : d);
-            const float2 c = (isP || z) ? make_float2(0.0f, 0.0f) : Mul64(S[k], r);
+            const float2 c = (isP || z) ? make_float2(0.0f, 0.0f) : Mul64(sk, r);
 #pragma unroll
             for (int j = 0; j < NL; ++j)
                 if (j != k) S[j] = SubMul(S[j], c, s_row[g][j]);
             if (!z)
             {
-                S[k] = isP ? make_float2(1.0f, 0.0f) : make_float2(-c.x, -c.y);
+                SetCol(S, k, isP ? make_float2(1.0f, 0.0f) : make_float2(-c.x, -c.y));
                 if (isP) sigma = r;
             }
             if (isP) used = true;
@@ -887,7 +908,7 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
     const int pr = rowLive ? s_perm[g][lane] : -1;
 #pragma unroll
     for (int j = 0; j < NL; ++j) X[j] = make_float2(j == pr ? 1.0f : 0.0f, 0.0f);
-#pragma unroll
+#pragma unroll(NL <= 32 ? NL : 1)
     for (int q = 0; q < NL; ++q)
     {
         if (q < n)
@@ -901,21 +922,21 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (lane > q && rowLive)
             {
-                const float2 l = LU[q];
+                const float2 l = GetCol(LU, q);
 #pragma unroll
                 for (int j = 0; j < NL; ++j) X[j] = SubMul(X[j], l, s_row[g][j]);
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
-#pragma unroll
+#pragma unroll(NL <= 32 ? NL : 1)
     for (int q = NL - 1; q >= 0; --q)
     {
         if (q < n)
         {
             if (lane == q)
             {
-                const float2 d = LU[q];
+                const float2 d = GetCol(LU, q);
                 const bool z = d.x == 0.0f && d.y == 0.0f;
                 const double2 r = Recip64(z ? make_float2(1.0f, 0.0f) : d);
 #pragma unroll
@@ -929,7 +950,7 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (lane < q && rowLive)
             {
-                const float2 uq = LU[q];
+                const float2 uq = GetCol(LU, q);
 #pragma unroll
                 for (int j = 0; j < NL; ++j) X[j] = SubMul(X[j], uq, s_row[g][j]);
             }
