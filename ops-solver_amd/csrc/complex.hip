@@ -254,6 +254,277 @@ __global__ __launch_bounds__(128) void cpanel_swap_trsm_kernel(float2* __restric
     if (t < wact) col[c0 + t] = s_x[t];
 }
 
+// ---------------------------------------------------------------------------------------
+// Tall complex leaf over several one-wave workgroups (the design of leaf_multi_kernel in
+// getrf.hip): rows [c0, m) split over P = gridDim.x workgroups of 64*R rows; per column each
+// wave reduces its candidates to one 64-bit key (pivot score, then the lowest position),
+// publishes its candidate's row (two tagged 64-bit words per complex value: real, imag) and
+// the key, polls the P keys, fetches the winning row and eliminates with the FP64
+// reciprocal and SubMul order of cleaf_kernel. The interchanges are written to leafmap for
+// cpanel_swap_trsm_kernel, exactly as cleaf_kernel does. Spins are bounded; on a timeout the
+// workgroup raises *err and fills its rows with NaN.
+// ---------------------------------------------------------------------------------------
+constexpr int kCSlotWords = 64;      // 2 key words + 2 words per complex column (W <= 16)
+constexpr int kCMaxGroups = 128;
+constexpr size_t kCSlotBytes = 2ull * kCMaxGroups * kCSlotWords * sizeof(unsigned long long);
+constexpr unsigned kCLeafSpin = 1u << 20;
+
+template <int CTRL>
+__device__ inline unsigned long long CDpp64(unsigned long long v)
+{
+    const int vlo = static_cast<int>(v & 0xffffffffull), vhi = static_cast<int>(v >> 32);
+    const int lo = __builtin_amdgcn_update_dpp(vlo, vlo, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(vhi, vhi, CTRL, 0xF, 0xF, false);
+    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+}
+__device__ inline unsigned long long CLane64(unsigned long long v, int l)
+{
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), l);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), l);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+__device__ inline unsigned long long CWaveMax64(unsigned long long v)
+{
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    v = mx(v, CDpp64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, CDpp64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, CDpp64<0x124>(v));  // row_ror:4
+    v = mx(v, CDpp64<0x128>(v));  // row_ror:8
+    return mx(mx(CLane64(v, 0), CLane64(v, 16)), mx(CLane64(v, 32), CLane64(v, 48)));
+}
+__device__ inline unsigned long long CLoadSlot(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void CStoreSlot(unsigned long long* p, unsigned tag, unsigned payload)
+{
+    __hip_atomic_store(p, (static_cast<unsigned long long>(tag) << 32) | payload, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int R, int W>
+__global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A, int lda, int m, int c0, int wact,
+                                                         int pivoting, int newPanel, int* __restrict__ ipiv,
+                                                         int* __restrict__ info, int* __restrict__ rowsrc,
+                                                         int* __restrict__ leafmap, unsigned long long* slots,
+                                                         unsigned tagBase, int* err)
+{
+    constexpr int T = 64;
+    static_assert(2 * W + 2 <= kCSlotWords && 2 * W <= T, "slot layout");
+    __shared__ float s_row[2 * W];   // this wave's candidate row (re, im interleaved)
+    __shared__ float s_prow[2 * W];  // the pivot row
+    __shared__ int s_bpos[W];
+    __shared__ int s_count;
+    const int lane = threadIdx.x;
+    const int P = gridDim.x, g = blockIdx.x;
+    const int mp = m - c0;
+    float2* Ap = A + static_cast<size_t>(c0) * lda + c0;
+    const int rbase = g * T * R;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+
+    float2 a[R][W];
+    int pos[R], srcv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int lr = rbase + lane + i * T;
+        const bool valid = lr < mp;
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            a[i][j] = (valid && j < wact) ? Ap[static_cast<size_t>(j) * lda + lr] : make_float2(0.0f, 0.0f);
+        pos[i] = valid ? lr : INT_MAX;
+        // read before any workgroup stores (stores follow the last column's exchange)
+        srcv[i] = valid ? (newPanel ? c0 + lr : rowsrc[c0 + lr]) : 0;
+    }
+    if (lane == 0) s_count = 0;
+    int zeroCol = 0;
+    bool aborted = false;
+
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+    {
+        if (j < wact && !aborted)  // uniform (one wave)
+        {
+            const unsigned tag = tagBase + static_cast<unsigned>(j);
+            unsigned long long* base = slots + static_cast<size_t>(j & 1) * kCMaxGroups * kCSlotWords;
+            unsigned long long* mine = base + g * kCSlotWords;
+            unsigned long long best = 0ull;
+            int bi = -1;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const bool cand = pos[i] != INT_MAX && (pivoting ? pos[i] >= j : pos[i] == j);
+                const unsigned long long k =
+                    cand ? (static_cast<unsigned long long>(PivotScore(a[i][j])) << 32) | static_cast<unsigned>(~pos[i])
+                         : 0ull;
+                bi = k > best ? i : bi;
+                best = k > best ? k : best;
+            }
+            const unsigned long long top = CWaveMax64(best);
+            if (best == top && bi >= 0)
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (i == bi)
+#pragma unroll
+                        for (int k = 0; k < W; ++k)
+                        {
+                            s_row[2 * k] = a[i][k].x;
+                            s_row[2 * k + 1] = a[i][k].y;
+                        }
+            __syncthreads();
+            if (lane < 2 * wact)
+                CStoreSlot(mine + 2 + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
+            else if (lane == 2 * wact)
+                CStoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
+            else if (lane == 2 * wact + 1)
+                CStoreSlot(mine + 1, tag, static_cast<unsigned>(top));
+            // the P keys: lane l reads workgroups l and l + 64
+            unsigned long long key0 = 0ull, key1 = 0ull;
+            bool ok = false;
+            for (unsigned spins = 0; spins < kCLeafSpin; ++spins)
+            {
+                unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
+                if (lane < P)
+                {
+                    hi0 = CLoadSlot(base + lane * kCSlotWords);
+                    lo0 = CLoadSlot(base + lane * kCSlotWords + 1);
+                }
+                if (lane + 64 < P)
+                {
+                    hi1 = CLoadSlot(base + (lane + 64) * kCSlotWords);
+                    lo1 = CLoadSlot(base + (lane + 64) * kCSlotWords + 1);
+                }
+                const bool good0 =
+                    lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
+                const bool good1 = lane + 64 >= P ||
+                                   (static_cast<unsigned>(hi1 >> 32) == tag && static_cast<unsigned>(lo1 >> 32) == tag);
+                key0 = lane < P ? (hi0 << 32) | (lo0 & 0xffffffffull) : 0ull;
+                key1 = lane + 64 < P ? (hi1 << 32) | (lo1 & 0xffffffffull) : 0ull;
+                if (__all(good0 && good1))
+                {
+                    ok = true;
+                    break;
+                }
+            }
+            const unsigned long long gtop = CWaveMax64(key0 > key1 ? key0 : key1);
+            const unsigned long long hit0 = __ballot(ok && lane < P && key0 == gtop);
+            const unsigned long long hit1 = __ballot(ok && lane + 64 < P && key1 == gtop);
+            const int winner = hit0 != 0ull   ? __ffsll(static_cast<long long>(hit0)) - 1
+                               : hit1 != 0ull ? 64 + __ffsll(static_cast<long long>(hit1)) - 1
+                                              : 0;
+            unsigned long long rw = 0ull;
+            for (unsigned spins = 0; ok; ++spins)
+            {
+                rw = lane < 2 * wact ? CLoadSlot(base + winner * kCSlotWords + 2 + lane)
+                                     : static_cast<unsigned long long>(tag) << 32;
+                if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
+                if (spins >= kCLeafSpin) ok = false;
+            }
+            if (!ok)
+            {
+                if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                aborted = true;
+                continue;
+            }
+            if (lane < 2 * W) s_prow[lane] = __uint_as_float(static_cast<unsigned>(rw));
+            __syncthreads();
+            const int bpos = gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                if (pos[i] == bpos)
+                    pos[i] = j;
+                else if (pos[i] == j)
+                    pos[i] = bpos;
+            }
+            if (lane == 0)
+            {
+                s_bpos[j] = bpos;
+                if (g == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
+            }
+            const float2 d = make_float2(s_prow[2 * j], s_prow[2 * j + 1]);
+            const bool zero = d.x == 0.0f && d.y == 0.0f;
+            if (zero && zeroCol == 0) zeroCol = c0 + j + 1;
+            const double2 rinv = Recip64(zero ? make_float2(1.0f, 0.0f) : d);
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+            {
+                const bool below = pos[i] > j && pos[i] != INT_MAX && !zero;
+                const float2 q = Mul64(a[i][j], rinv);
+                const float2 l = below ? q : make_float2(0.0f, 0.0f);
+                a[i][j] = below ? q : a[i][j];
+#pragma unroll
+                for (int k = j + 1; k < W; ++k)
+                    a[i][k] = SubMul(a[i][k], l, make_float2(s_prow[2 * k], s_prow[2 * k + 1]));
+            }
+        }
+    }
+    if (aborted)
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < W; ++j) a[i][j] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int lr = rbase + lane + i * T;
+        if (lr < mp)
+        {
+            const int dst = pos[i];
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (j < wact) Ap[static_cast<size_t>(j) * lda + dst] = a[i][j];
+            rowsrc[c0 + dst] = srcv[i];
+        }
+    }
+    if (g != 0) return;
+    if (lane == 0 && zeroCol != 0 && *info == 0) *info = zeroCol;
+    if (aborted)
+    {
+        if (lane == 0) leafmap[0] = 0;
+        return;
+    }
+    __syncthreads();
+    // workgroup 0 lists the moved rows for cpanel_swap_trsm_kernel (undoing the interchange
+    // sequence from each touched position, as leaf_multi_kernel does)
+    {
+        int p = -1;
+        if (lane < W)
+            p = lane < wact ? lane : -1;
+        else if (lane - W < wact)
+        {
+            const int jj = lane - W;
+            p = s_bpos[jj] >= wact ? s_bpos[jj] : -1;
+            for (int t = 0; t < jj && p >= 0; ++t)
+                if (s_bpos[t] == p) p = -1;
+        }
+        if (p >= 0)
+        {
+            int x = p;
+            for (int jj = wact - 1; jj >= 0; --jj)
+            {
+                const int b = s_bpos[jj];
+                x = x == jj ? b : (x == b ? jj : x);
+            }
+            if (x != p)
+            {
+                const int slot = atomicAdd(&s_count, 1);
+                leafmap[1 + 2 * slot] = c0 + p;
+                leafmap[2 + 2 * slot] = c0 + x;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane == 0) leafmap[0] = s_count;
+}
+
+__global__ void cclear_slots_kernel(unsigned long long* slots, int words, int* err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) slots[i] = 0ull;
+    if (i == 0) *err = 0;
+}
+
 __global__ __launch_bounds__(1024) void ccompact_moves_kernel(int m, int j0, const int* __restrict__ rowsrc,
                                                               int* __restrict__ moved)
 {
@@ -534,16 +805,33 @@ hipError_t CTrsmBlock(int K, int N, const float2* T, int ldt, bool lower, bool u
     return hipSuccess;
 }
 
+// Workspace: ints [0, m) row sources, the panel's moved-row list (4 CNB + 8), the leaf's
+// map (136); then, 8-byte aligned, the exchange slots of the multi-workgroup leaf and its
+// timeout flag.
+constexpr int kCLeafMultiRows = 2048;  // panels taller than this spread their leaves
 struct CWork
 {
     int* rowsrc;
     int* moved;
     int* leafmap;
+    unsigned long long* slots;
+    int* err;
+    unsigned* epoch;  // host counter: tag base of the next multi-workgroup leaf
 };
+inline size_t CWorkInts(int m) { return static_cast<size_t>(m) + 4 * CNB + 8 + 1 + 2 * 64 + 7; }
 CWork CLayout(void* base, int m)
 {
     int* p = static_cast<int*>(base);
-    return {p, p + m, p + m + 4 * CNB + 8};
+    CWork w;
+    w.rowsrc = p;
+    w.moved = p + m;
+    w.leafmap = p + m + 4 * CNB + 8;
+    uintptr_t q = reinterpret_cast<uintptr_t>(p + CWorkInts(m));
+    q = (q + 7) & ~static_cast<uintptr_t>(7);
+    w.slots = reinterpret_cast<unsigned long long*>(q);
+    w.err = reinterpret_cast<int*>(q + kCSlotBytes);
+    w.epoch = nullptr;
+    return w;
 }
 
 template <int T, int R, int W>
@@ -557,6 +845,7 @@ hipError_t LaunchCLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int 
 
 int CLeafWidth(int mp)
 {
+    if (mp > kCLeafMultiRows) return 16;
     if (mp <= 512) return 16;
     if (mp <= 4096) return 8;
     if (mp <= 8192) return 4;
@@ -567,6 +856,16 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
                  const CWork& w, hipStream_t stream)
 {
     const int mp = m - c0;
+    if (mp > kCLeafMultiRows)
+    {
+        constexpr int R = 2, W = 16;
+        const int groups = (mp + 64 * R - 1) / (64 * R);
+        if (groups > kCMaxGroups || wact > W || w.epoch == nullptr) return hipErrorInvalidValue;
+        const unsigned tagBase = (*w.epoch)++ * 64u;
+        hipLaunchKernelGGL((cleaf_multi_kernel<R, W>), dim3(groups), dim3(64), 0, stream, A, lda, m, c0, wact,
+                           pivoting, newPanel, ipiv, info, w.rowsrc, w.leafmap, w.slots, tagBase, w.err);
+        return hipGetLastError();
+    }
 #define CLEAF(T, R, W) LaunchCLeaf<T, R, W>(A, lda, m, j0, c0, wact, pivoting, newPanel, ipiv, info, w, stream)
     if (mp <= 256) return CLEAF(256, 1, 16);
     if (mp <= 512) return CLEAF(512, 1, 16);
@@ -590,14 +889,21 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
 size_t ComplexGetrfWorkspaceBytes(int m, int n)
 {
     (void)n;
-    return (static_cast<size_t>(std::max(m, 1)) + 4 * CNB + 8 + 1 + 2 * 64 + 7) * sizeof(int);
+    return CWorkInts(std::max(m, 1)) * sizeof(int) + 8 + kCSlotBytes + 16;
 }
 
 hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream)
 {
     if (m > 16384) return hipErrorInvalidValue;
     CWork w = CLayout(work, m);
+    unsigned epoch = 1;
+    w.epoch = &epoch;
     FillInfo(info, 1, 0, stream);
+    if (m > kCLeafMultiRows)
+    {
+        constexpr int words = static_cast<int>(kCSlotBytes / sizeof(unsigned long long));
+        hipLaunchKernelGGL(cclear_slots_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, w.slots, words, w.err);
+    }
     const int kmax = std::min(m, n);
     const int pivoting = ipiv != nullptr;
     for (int j0 = 0; j0 < kmax; j0 += CNB)

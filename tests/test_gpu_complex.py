@@ -59,7 +59,8 @@ def _lu_residual(a, lu, piv):
 
 
 @pytest.mark.parametrize("m,n,seed", [(64, 64, 1), (300, 300, 2), (129, 129, 3), (1000, 100, 4), (800, 64, 5),
-                                      (1000, 40, 6), (5, 9, 7), (1, 1, 8), (700, 700, 9)])
+                                      (1000, 40, 6), (5, 9, 7), (1, 1, 8), (700, 700, 9), (2200, 300, 10),
+                                      (5000, 140, 11), (9000, 33, 12)])
 def test_cgetrf_parity(torch, handle, m, n, seed):
     rng = np.random.default_rng(seed)
     a = _crand(rng, m, n)
@@ -71,6 +72,21 @@ def test_cgetrf_parity(torch, handle, m, n, seed):
     np.testing.assert_array_equal(piv[:upto], ref_piv[:upto])
     # complex_solver_test.cpp:56-84: ||PA-LU||_1 / (n ||A||_1 eps) <= 5
     assert O.lu_backward_error(a, lu, piv) / EPS32 <= 5
+
+
+def test_cgetrf_tall_zero_column(torch, handle):
+    """Tall panels (leaves spread over several workgroups): an exactly zero column gives
+    info at that column, as the reference's first zero pivot."""
+    m, n = 4500, 48
+    rng = np.random.default_rng(13)
+    a = _crand(rng, m, n)
+    a[:, 21] = 0
+    st, lu, piv, info = _cgetrf(torch, handle, a)
+    _, ref_piv, ref_info, margin = O.cgetrf(a, margins=True)
+    assert st == A.SUCCESS and info == ref_info == 22
+    bad = np.nonzero(margin[:21] < 1e-4)[0]
+    upto = int(bad[0]) if len(bad) else 21
+    np.testing.assert_array_equal(piv[:upto], ref_piv[:upto])
 
 
 def test_complex_pivot_kats(torch, handle):
