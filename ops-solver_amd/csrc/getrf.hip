@@ -347,7 +347,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                                                         unsigned tagBase, int* err)
 {
     constexpr int T = 64;
-    static_assert(W + 2 <= kSlotWords && 2 * W <= T, "slot layout");
+    static_assert(W + 2 <= kSlotWords && W <= T, "slot layout");
     constexpr int MOVES = 2 * W;
     __shared__ float s_row[W];
     __shared__ float s_prow[2][W];  // pivot rows of columns j (slot j % 2) and j - 1
@@ -545,15 +545,16 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
 
     // The moved rows, from the interchange sequence (identical in every workgroup): the row
     // that ends at position p came from the position found by undoing the interchanges in
-    // reverse. Candidates: positions 0..wact-1 (lanes 0..W-1) and the pivot positions
-    // beyond them (lanes W..2W-1, first occurrence only).
+    // reverse. Candidates: positions 0..wact-1 (items 0..W-1) and the pivot positions
+    // beyond them (items W..2W-1, first occurrence only).
+    for (int it = lane; it < 2 * W; it += T)
     {
         int p = -1;
-        if (lane < W)
-            p = lane < wact ? lane : -1;
-        else if (lane - W < wact)
+        if (it < W)
+            p = it < wact ? it : -1;
+        else if (it - W < wact)
         {
-            const int jj = lane - W;
+            const int jj = it - W;
             p = s_bpos[jj] >= wact ? s_bpos[jj] : -1;
             for (int t = 0; t < jj && p >= 0; ++t)
                 if (s_bpos[t] == p) p = -1;
@@ -581,27 +582,32 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
     // before it is scattered.
     const int cnt = s_count;
     const int left = c0 - j0, others = left + (panelEnd - c0 - wact);
-    constexpr int SWB = 4;
+    constexpr int SWB = 4, EPL = (MOVES + T - 1) / T;  // moved entries per lane
     for (int q0 = g; q0 < others; q0 += SWB * P)  // uniform trip count
     {
-        float v[SWB];
+        float v[SWB][EPL];
         int cc[SWB];
 #pragma unroll
         for (int bb = 0; bb < SWB; ++bb)
         {
             const int q = q0 + bb * P;
-            cc[bb] = -1;
-            v[bb] = 0.0f;
-            if (q < others && lane < cnt)
+            cc[bb] = q < others ? (q < left ? j0 + q : c0 + wact + (q - left)) : -1;
+#pragma unroll
+            for (int u = 0; u < EPL; ++u)
             {
-                cc[bb] = q < left ? j0 + q : c0 + wact + (q - left);
-                v[bb] = A[static_cast<size_t>(cc[bb]) * lda + s_msrc[lane]];
+                const int e = lane + u * T;
+                v[bb][u] = (cc[bb] >= 0 && e < cnt) ? A[static_cast<size_t>(cc[bb]) * lda + s_msrc[e]] : 0.0f;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int bb = 0; bb < SWB; ++bb)
-            if (cc[bb] >= 0) A[static_cast<size_t>(cc[bb]) * lda + s_mdst[lane]] = v[bb];
+#pragma unroll
+            for (int u = 0; u < EPL; ++u)
+            {
+                const int e = lane + u * T;
+                if (cc[bb] >= 0 && e < cnt) A[static_cast<size_t>(cc[bb]) * lda + s_mdst[e]] = v[bb][u];
+            }
     }
 }
 
@@ -714,7 +720,7 @@ int MultiLeafRows()
 }
 inline bool UseMultiLeaf(int mp) { return MultiLeafRows() > 0 && mp > MultiLeafRows(); }
 
-constexpr int kMultiT = 64, kMultiR = 2, kMultiW = 32;  // one wave of 128 rows per workgroup, 32 columns
+constexpr int kMultiT = 64, kMultiR = 2, kMultiW = 64;  // one wave of 128 rows per workgroup, 64 columns
 
 hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
                      int* ipiv, int* info, const Work& w, hipStream_t stream)
