@@ -1,20 +1,31 @@
-// Uneven batched LU solve (getrf + getrs per matrix) in one persistent launch (EXTENSION;
-// the reference has no batched getrf/getrs).
+// Uneven batched LU solve (getrf + getrs per matrix), EXTENSION: the reference has no
+// batched getrf/getrs.
 //
-// Plan: matrices are binned by order into 64 / 128 / 256 row classes; a work item fills one
-// 512-thread workgroup (8 matrices of n <= 64 with one wave each, 4 of n <= 128 with two
-// waves each, or 1 of n <= 256 with two threads per row), and every thread keeps at most
-// 128 matrix values in VGPRs. Items are queued largest class first, larger orders first
-// (longest-processing-time order). A persistent grid pulls items from a device counter
-// until the queue is empty, so every workgroup reaches the exit. The autotuner picks the
-// grid size at plan time.
+// Plan: matrices are binned by order into four classes, each with its own kernel so that
+// each class gets the register budget (and occupancy) it needs:
+//   n <= 64:        one wave per matrix, 8 matrices per 512-thread workgroup, a persistent
+//                   grid pulling items from a device counter (longest first);
+//   65..128:        column-cyclic kernel, 4 waves, 2 rows per lane;
+//   129..192:       column-cyclic kernel, 8 waves, 3 rows per lane;
+//   193..256:       column-cyclic kernel, 8 waves, 4 rows per lane.
+// The classes are launched largest first, each class ordered by decreasing n
+// (longest-processing-time order).
 //
-// Per matrix: rows live in registers (position-tracked partial pivoting exactly as the
-// register leaf of getrf.hip: largest |a|, lowest position on ties, reciprocal scaling,
-// fused rank-1 updates); per column a 64-bit key argmax (DPP in the wave, one slot per wave
-// through LDS) and two barriers (one wave: wave-level only); the right-hand sides ride
-// along with their rows, then a back substitution in 32-column blocks. Outputs: packed LU
-// (rows stored at their final positions), 1-based ipiv, X, info (first zero pivot).
+// Column-cyclic kernel (one matrix per workgroup): lane l of every wave holds rows
+// l, l + 64, ... of the matrix; wave w holds columns w, w + NW, w + 2 NW, ... of the augmented
+// matrix [A | B]. The pivot search of column k runs entirely inside its owner wave (a 64-bit
+// key argmax: largest |a|, lowest position on ties, NaN never wins); the owner publishes the
+// pivot position and the multipliers of its rows through a ring of LDS entries guarded by
+// per-entry flags, and every wave updates its own columns with the pivot row read by
+// v_readlane from its own registers. No workgroup barrier runs during the elimination: a
+// wave waits only for the column it needs, and the owner of column k + 1 updates that
+// column first and publishes its pivot before finishing the update of column k
+// (look-ahead). Rows keep their registers; the position each occupies is tracked
+// identically in every wave (implicit pivoting, as in the register leaves of the dense
+// factorization). The right-hand sides are extra columns, so the elimination also forward
+// substitutes them; the back substitution is done by one wave from the stored U.
+// Outputs: packed LU (rows at their final positions), 1-based ipiv, X, info (first zero
+// pivot).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +47,8 @@ constexpr int kBS = 32;      // back-substitution block
 
 struct Item
 {
-    int bin;     // 0: n<=64 (8 per item), 1: n<=128 (4 per item), 2: n<=256 (1 per item)
-    int first;   // index into the plan's matrix order
-    int count;   // matrices in this item
+    int first;  // index into the plan's order of the n <= 64 class
+    int count;  // matrices in this item (<= 8)
 };
 
 __device__ inline unsigned long long KeyOf(float v, int pos)
@@ -364,13 +374,7 @@ __device__ __attribute__((always_inline)) void solve_group(int n, int nLoop, int
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
-union VbLds
-{
-    GroupLds<64, 1> s[8];
-    GroupLds<128, 1> m[4];
-    GroupLds<256, 2> l;
-};
-
+// n <= 64: 8 matrices per workgroup, one wave each (wave-level synchronization only).
 __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ items, int numItems,
                                                        const int* __restrict__ order, const int* __restrict__ nArr,
                                                        const long long* __restrict__ pivOff, int nrhs,
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ 
                                                        int* __restrict__ ipiv, int* __restrict__ info,
                                                        int* __restrict__ counter)
 {
-    __shared__ VbLds lds;
+    __shared__ GroupLds<64, 1> lds[8];
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (;;)
@@ -389,34 +393,534 @@ __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ 
         __syncthreads();
         if (it >= numItems) break;  // every workgroup reaches this exit once the queue drains
         const Item item = items[it];
-        if (item.bin == 0)  // 8 matrices, one wave64 each, wave-level synchronization only
-        {
-            const int g = tid >> 6, t = tid & 63;
-            const bool live = g < item.count;
-            const int mi = live ? order[item.first + g] : 0;
-            const int n = live ? nArr[mi] : 0;
-            solve_group<64, 1, true>(n, n, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
-                                     ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, lds.s[g]);
-        }
-        else if (item.bin == 1)  // 4 matrices, 128 threads each, in lockstep (largest first)
-        {
-            const int g = tid >> 7, t = tid & 127;
-            const bool live = g < item.count;
-            const int mi = live ? order[item.first + g] : 0;
-            const int nLead = nArr[order[item.first]];
-            solve_group<128, 1, false>(live ? nArr[mi] : 0, nLead, nrhs, live ? Aarr[mi] : nullptr,
-                                       live ? Barr[mi] : nullptr, ipiv + (live ? pivOff[mi] : 0), info + mi, t, live,
-                                       lds.m[g]);
-        }
-        else  // 1 matrix, two threads per row
-        {
-            const int mi = order[item.first];
-            const int n = nArr[mi];
-            solve_group<256, 2, false>(n, n, nrhs, Aarr[mi], Barr[mi], ipiv + pivOff[mi], info + mi, tid, true,
-                                       lds.l);
-        }
+        const int g = tid >> 6, t = tid & 63;
+        const bool live = g < item.count;
+        const int mi = live ? order[item.first + g] : 0;
+        const int n = live ? nArr[mi] : 0;
+        solve_group<64, 1, true>(n, n, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
+                                 ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, lds[g]);
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// Column-cyclic kernel for 65 <= n <= 256.
+
+constexpr int kRing = 16;  // LDS ring entries of published pivot columns
+
+template <int RPL, int NW, int CPW>
+struct CycLds
+{
+    float4 l[kRing][64];            // multipliers of the published column, by lane (row slots .x..)
+    unsigned long long tag[kRing];  // (column index + 1) | pivot position, lane, slot << 32
+    int done[NW];                   // columns whose pivot information each wave has taken
+    int zmin;                       // first zero pivot (1-based) over the owners
+    int abort;                      // set when a wait exceeded its bound: every wave then runs out
+    float4 prow[NW][2][CPW / 4];    // per wave: pivot row of the current / the deferred step
+    float y[kNrhsMax][RPL * 64];
+};
+
+constexpr int kCycSpin = 1 << 20;  // bound on the sleeps of one wait (~30 ms)
+
+// True when the workgroup has given up (another wave's wait ran out) or this wait has run
+// out, in which case it raises the flag; info then reports -1 for the matrix.
+__device__ inline bool Aborted(int& flag, int it)
+{
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+        return true;
+    if (it < kCycSpin) return false;
+    __hip_atomic_store(&flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return true;
+}
+
+__device__ inline float Rl(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// The pivot row's values in this wave's column slots, written to the wave's LDS row buffer
+// by the lane that holds the row (lane lp, slot sp; call under lane == lp) for the chunks of
+// four slots below nv; every lane then reads them back as a broadcast. This replaces one
+// v_readlane per slot and keeps the branch over sp away from the update of the slot array.
+template <int S, int RPL, int CPW>
+__device__ __attribute__((always_inline)) inline void PutPivotRow(const float (&a)[RPL][CPW], int sp, int nv,
+                                                                  float4* __restrict__ row)
+{
+    if constexpr (S < RPL)
+    {
+        if (S == RPL - 1 || sp == S)
+        {
+#pragma unroll
+            for (int K = 0; K < CPW; K += 4)
+                if (K < nv) row[K / 4] = make_float4(a[S][K], a[S][K + 1], a[S][K + 2], a[S][K + 3]);
+        }
+        else
+            PutPivotRow<S + 1>(a, sp, nv, row);
+    }
+}
+
+// Rank-1 update of the unfinished column slots [0, nv) of a wave, a[i][kk] -= l[i] * u[kk],
+// in chunks of four slots (chunks at or beyond nv are skipped). Slots >= nv hold no live
+// column (finished columns are already stored), so the chunk that straddles nv needs no
+// mask.
+template <int RPL, int CPW>
+__device__ __attribute__((always_inline)) inline void UpdateSlots(float (&a)[RPL][CPW], const float (&l)[RPL], int nv,
+                                                                  const float4* __restrict__ row)
+{
+#pragma unroll
+    for (int K = 0; K < CPW; K += 4)
+    {
+        if (K < nv)
+        {
+            const float4 u4 = row[K / 4];
+            const float u[4] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) a[i][K + t] = fmaf(-l[i], u[t], a[i][K + t]);
+        }
+    }
+}
+
+// The look-ahead step's update of slots 1.. with two rank-1 updates, in order: the deferred
+// one of the previous column (ld, pivot row in rowD; zero when nothing was deferred) and the
+// current one (lc, pivot row in rowC), while the slots rotate down by one (slot 0, the
+// column just pivoted, leaves the array), so the wave's next pivot column is always slot 0
+// and no slot is addressed by a run-time index. The current pivot row has not received the
+// deferred update yet, so its values are corrected first with the same fused multiply-add
+// the row itself would have seen (ldp = its deferred multiplier): the result is
+// bit-identical to applying the two updates one after the other.
+template <int RPL, int CPW>
+__device__ __attribute__((always_inline)) inline void RotSlots(float (&a)[RPL][CPW], const float (&ld)[RPL], float ldp,
+                                                               const float (&lc)[RPL], int nv,
+                                                               const float4* __restrict__ rowD,
+                                                               const float4* __restrict__ rowC)
+{
+#pragma unroll
+    for (int K = 0; K < CPW; K += 4)
+    {
+        if (K < nv)
+        {
+            const float4 d4 = rowD[K / 4], c4 = rowC[K / 4];
+            const float ud[4] = {d4.x, d4.y, d4.z, d4.w};
+            const float uc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                const int kk = K + t;
+                if (kk == 0) continue;
+                const float ck = fmaf(-ldp, ud[t], uc[t]);
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) a[i][kk - 1] = fmaf(-lc[i], ck, fmaf(-ld[i], ud[t], a[i][kk]));
+            }
+        }
+    }
+}
+
+// v[sp] of lane lp (sp, lp uniform): every slot is read by v_readlane and the choice is made
+// on the scalar side (a select over the array itself would be turned into an indexed
+// load from scratch memory).
+template <int RPL>
+__device__ inline float RlSel(const float (&v)[RPL], int sp, int lp)
+{
+    float r = Rl(v[0], lp);
+#pragma unroll
+    for (int i = 1; i < RPL; ++i)
+    {
+        const float t = Rl(v[i], lp);
+        r = __int_as_float(sp == i ? __float_as_int(t) : __float_as_int(r));
+    }
+    return r;
+}
+
+template <int CTRL>
+__device__ inline unsigned DppU32(unsigned v)
+{
+    return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+// Wave-wide max (MAX = true) or min of a 32-bit unsigned value, returned uniform.
+template <bool MAX>
+__device__ inline unsigned WaveReduceU32(unsigned v)
+{
+    auto op = [](unsigned a, unsigned b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = op(v, DppU32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, DppU32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, DppU32<0x124>(v));  // row_ror:4
+    v = op(v, DppU32<0x128>(v));  // row_ror:8
+    const unsigned r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const unsigned r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
+// Pivot of column k from the owner's values col (rows by slot): among the rows not yet
+// pivoted, the largest |a| and, on ties, the lowest position (NaN never beats a number; a
+// column of zeros and NaNs takes its lowest position). A 32-bit key orders the magnitudes
+// (0: not a candidate, 1: NaN, else bits(|a|) + 2) and is reduced across the wave in one
+// pass; ties across lanes are rare and resolved by a second (min position) pass. Then the
+// multipliers of the rows below the pivot after the interchange (reciprocal scaling when
+// |d| >= FLT_MIN, division otherwise) replace their values in col.
+template <int RPL>
+__device__ __attribute__((always_inline)) inline void PivotColumn(float (&col)[RPL], const int (&pos)[RPL], int k,
+                                                                  float (&l)[RPL], int& bpos, int& lp, int& sp,
+                                                                  float& d)
+{
+    unsigned bkey = 0u;
+    int bp = INT_MAX, bi = 0;
+    float bv = 0.0f;
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)  // branch-free: bitwise logic and selects only
+    {
+        const bool cand = (pos[i] >= k) & (pos[i] != INT_MAX);
+        const float v = col[i];
+        unsigned key = (__float_as_uint(v) & 0x7fffffffu) + 2u;
+        key = v != v ? 1u : key;
+        key = cand ? key : 0u;
+        const bool better = (key > bkey) | ((key == bkey) & (pos[i] < bp));
+        bkey = better ? key : bkey;
+        bp = better ? pos[i] : bp;
+        bi = better ? i : bi;
+        bv = better ? v : bv;
+    }
+    const unsigned top = WaveReduceU32<true>(bkey);
+    const unsigned long long hit = __builtin_amdgcn_ballot_w64(bkey == top);
+    if (__builtin_popcountll(hit) == 1)
+        lp = __builtin_ctzll(hit);
+    else  // several lanes hold the largest magnitude: the lowest position wins
+    {
+        const unsigned mp = WaveReduceU32<false>(bkey == top ? static_cast<unsigned>(bp) : 0xffffffffu);
+        lp = __builtin_ctzll(__builtin_amdgcn_ballot_w64(bkey == top && static_cast<unsigned>(bp) == mp));
+    }
+    bpos = __builtin_amdgcn_readlane(bp, lp);
+    d = Rl(bv, lp);
+    sp = __builtin_amdgcn_readlane(bi, lp);
+    int np[RPL];
+    bool below[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i)
+    {
+        np[i] = pos[i] == bpos ? k : (pos[i] == k ? bpos : pos[i]);
+        below[i] = (np[i] > k) & (np[i] != INT_MAX) & (d != 0.0f);
+    }
+    if (fabsf(d) >= FLT_MIN)  // uniform
+    {
+        const float rcp = 1.0f / d;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) l[i] = below[i] ? col[i] * rcp : 0.0f;
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) l[i] = below[i] ? col[i] / d : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) col[i] = (below[i] & (l[i] != 0.0f)) ? l[i] : col[i];
+}
+
+// Back substitution over the columns k of position segment S (64 S <= k < 64 S + 64, k < n),
+// last to first, then the segments below. Lane l holds y at positions l + 64 i; column k of
+// U (rows 0 .. 64 S + 63, slots 0..S) is read from the stored LU with a prefetch of D columns
+// in registers. x_k = y_k * (1 / U_kk): the reciprocal depends only on U, so it stays off
+// the dependent chain that runs through the y values.
+template <int S, int RPL>
+__device__ __attribute__((always_inline)) inline void BackSubSeg(const float* __restrict__ A, int n, int lane,
+                                                                 float (&y)[kNrhsMax][RPL])
+{
+    constexpr int D = 8;
+    const int kLo = 64 * S;
+    const int kHi = min(n, 64 * S + 64) - 1;
+    float u[D][S + 1];
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+    {
+        const int k = kHi - t;
+#pragma unroll
+        for (int i = 0; i <= S; ++i)
+            u[t][i] = (k >= kLo && lane + 64 * i < n) ? A[static_cast<size_t>(k) * n + lane + 64 * i] : 0.0f;
+    }
+    for (int k0 = kHi; k0 >= kLo; k0 -= D)
+    {
+#pragma unroll
+        for (int t = 0; t < D; ++t)
+        {
+            const int k = k0 - t;
+            if (k >= kLo)
+            {
+                const int kl = k - kLo;
+                const float r = 1.0f / Rl(u[t][S], kl);
+#pragma unroll
+                for (int q = 0; q < kNrhsMax; ++q)
+                {
+                    const float x = Rl(y[q][S], kl) * r;
+#pragma unroll
+                    for (int i = 0; i < S; ++i) y[q][i] = fmaf(-u[t][i], x, y[q][i]);
+                    y[q][S] = lane < kl ? fmaf(-u[t][S], x, y[q][S]) : (lane == kl ? x : y[q][S]);
+                }
+            }
+            const int kn = k - D;
+#pragma unroll
+            for (int i = 0; i <= S; ++i)
+                u[t][i] = (kn >= kLo && lane + 64 * i < n) ? A[static_cast<size_t>(kn) * n + lane + 64 * i] : 0.0f;
+        }
+    }
+    if constexpr (S > 0) BackSubSeg<S - 1, RPL>(A, n, lane, y);
+}
+
+template <int RPL, int NW>
+__global__ __launch_bounds__(64 * NW) void vb_cyclic_kernel(const int* __restrict__ order,
+                                                            const int* __restrict__ nArr,
+                                                            const long long* __restrict__ pivOff, int nrhs,
+                                                            float* const* __restrict__ Aarr,
+                                                            float* const* __restrict__ Barr, int* __restrict__ ipiv,
+                                                            int* __restrict__ info)
+{
+    constexpr int NMAX = 64 * RPL;
+    constexpr int CPW = ((NMAX + kNrhsMax + NW - 1) / NW + 3) / 4 * 4;  // column slots per wave
+    __shared__ CycLds<RPL, NW, CPW> L;
+    const int mi = order[blockIdx.x];
+    const int n = nArr[mi];
+    float* __restrict__ A = Aarr[mi];
+    float* __restrict__ B = Barr[mi];
+    int* __restrict__ piv = ipiv + pivOff[mi];
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    if (tid < kRing) L.tag[tid] = 0ull;
+    if (tid < NW) L.done[tid] = 0;
+    if (tid == 0)
+    {
+        L.zmin = INT_MAX;
+        L.abort = 0;
+    }
+    for (int t = tid; t < NW * (CPW / 4); t += 64 * NW)  // no deferred pivot row yet
+        L.prow[t / (CPW / 4)][1][t % (CPW / 4)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    float a[RPL][CPW];
+    int pos[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) pos[i] = lane + 64 * i < n ? lane + 64 * i : INT_MAX;
+#pragma unroll
+    for (int kk = 0; kk < CPW; ++kk)
+    {
+        const int j = w + NW * kk;
+        const float* src = j < n ? A + static_cast<size_t>(j) * n
+                                 : (j < n + nrhs ? B + static_cast<size_t>(j - n) * n : nullptr);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) a[i][kk] = (src != nullptr && pos[i] != INT_MAX) ? src[lane + 64 * i] : 0.0f;
+    }
+    __syncthreads();
+
+    int zero = 0;
+    float lc[RPL];  // multipliers of the current column (this lane's rows)
+    int bp = 0, lp = 0, sp = 0;
+
+    int ringFree = kRing - 1;  // largest column whose ring entry is known to be free
+    auto publish = [&](int k, const float (&l)[RPL], int bpos, int plane, int pslot)
+    {
+        for (int it = 0; k > ringFree; ++it)  // the entry's previous column must have been taken
+        {
+            int mn = INT_MAX;
+#pragma unroll
+            for (int v = 0; v < NW; ++v)
+                mn = min(mn, __hip_atomic_load(&L.done[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            ringFree = __builtin_amdgcn_readfirstlane(mn) + kRing - 1;
+            if (k <= ringFree) break;
+            if (Aborted(L.abort, it)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const int e = k % kRing;
+        float4 lv = make_float4(l[0], RPL > 1 ? l[RPL > 1 ? 1 : 0] : 0.0f, RPL > 2 ? l[RPL > 2 ? 2 : 0] : 0.0f,
+                                RPL > 3 ? l[RPL > 3 ? 3 : 0] : 0.0f);
+        L.l[e][lane] = lv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const unsigned meta = static_cast<unsigned>(bpos) | (static_cast<unsigned>(plane) << 10) |
+                              (static_cast<unsigned>(pslot) << 16);
+        if (lane == 0)
+        {
+            __hip_atomic_store(&L.tag[e], static_cast<unsigned long long>(k + 1) | (static_cast<unsigned long long>(meta) << 32),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            piv[k] = bpos + 1;
+        }
+    };
+
+    auto take = [&](int c)  // pivot information of column c from its owner
+    {
+        const int e = c % kRing;
+        unsigned long long tg = 0ull;
+        for (int it = 0;; ++it)
+        {
+            tg = __hip_atomic_load(&L.tag[e], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (__builtin_amdgcn_readfirstlane(static_cast<unsigned>(tg)) == static_cast<unsigned>(c + 1)) break;
+            if (Aborted(L.abort, it)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const unsigned meta = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(tg >> 32));
+        bp = meta & 1023u;
+        lp = (meta >> 10) & 63u;
+        sp = meta >> 16;
+        const float4 lv = L.l[e][lane];
+        lc[0] = lv.x;
+        if constexpr (RPL > 1) lc[RPL > 1 ? 1 : 0] = lv.y;
+        if constexpr (RPL > 2) lc[RPL > 2 ? 2 : 0] = lv.z;
+        if constexpr (RPL > 3) lc[RPL > 3 ? 3 : 0] = lv.w;
+    };
+    auto taken = [&](int c)  // the ring entry of column c may be reused
+    {
+        if (lane == 0) __hip_atomic_store(&L.done[w], c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto interchange = [&](int c)
+    {
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) pos[i] = pos[i] == bp ? c : (pos[i] == c ? bp : pos[i]);
+    };
+    float4* __restrict__ rowC = L.prow[w][0];  // this wave's copy of the current pivot row
+    float4* __restrict__ rowD = L.prow[w][1];  // ... and of the deferred step's pivot row
+    // A finished (pivoted) column goes to memory at once, by original row index; a pass at
+    // the end moves every row to its final position.
+    auto storeCol = [&](int j, const float (&col)[RPL])
+    {
+        float* dst = A + static_cast<size_t>(j) * n;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i)
+            if (pos[i] != INT_MAX) dst[lane + 64 * i] = col[i];
+    };
+
+    // Slot kk of this wave holds column w + NW * (kk + rot); the live ones are slots [0, nv).
+    int rot = 0;
+    int nv = (n + nrhs - w + NW - 1) / NW;
+    if (w == 0)  // column 0
+    {
+        float col[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) col[i] = a[i][0];
+        float d;
+        PivotColumn(col, pos, 0, lc, bp, lp, sp, d);
+        if (d == 0.0f) zero = 1;
+        publish(0, lc, bp, lp, sp);
+        storeCol(0, col);
+#pragma unroll
+        for (int kk = 1; kk < CPW; ++kk)
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) a[i][kk - 1] = a[i][kk];
+        ++rot;
+        --nv;
+    }
+    // Each pass: plain steps, then (when there is room) a step that updates only slot 0 (this
+    // wave's next pivot column) and defers the rest of its update, then the look-ahead step:
+    // it brings slot 0 up to date, pivots it and publishes it at raised wave priority, and
+    // only then applies the deferred and the current update to the other slots. The pivot
+    // chain from column to column thus never waits for a wave's bulk update.
+    int c = 0;
+    for (;;)
+    {
+        const int cla = c + ((w - 1 - c) % NW + NW) % NW;  // next c with (c + 1) % NW == w
+        const int stop = cla < n - 1 ? cla - 1 : n;
+        for (; c < stop; ++c)
+        {
+            if (c % NW != w) take(c);  // else: this wave published column c itself
+            taken(c);
+            interchange(c);
+            if (lane == lp) PutPivotRow<0>(a, sp, nv, rowC);
+            UpdateSlots(a, lc, nv, rowC);
+        }
+        if (c >= n) break;
+        float ld[RPL];
+        if (c < cla)  // c == cla - 1 (never this wave's own column, NW >= 4): slot 0 only
+        {
+            take(c);
+            taken(c);
+            interchange(c);
+            if (lane == lp) PutPivotRow<0>(a, sp, nv, rowD);
+            float col[RPL];
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) col[i] = a[i][0];
+            const float u = RlSel(col, sp, lp);
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) a[i][0] = fmaf(-lc[i], u, a[i][0]);
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) ld[i] = lc[i];
+            ++c;
+        }
+        else  // nothing deferred: rowD still holds its initial zeros (first pass of wave 1)
+        {
+#pragma unroll
+            for (int i = 0; i < RPL; ++i) ld[i] = 0.0f;
+        }
+        // c == cla < n - 1: column c + 1 is this wave's slot 0
+        __builtin_amdgcn_s_setprio(2);
+        take(c);
+        interchange(c);
+        float col[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) col[i] = a[i][0];
+        const float u = RlSel(col, sp, lp);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) col[i] = fmaf(-lc[i], u, col[i]);
+        float ln[RPL];
+        int bpn, lpn, spn;
+        float d;
+        PivotColumn(col, pos, c + 1, ln, bpn, lpn, spn, d);
+        if (d == 0.0f && zero == 0) zero = c + 2;
+        publish(c + 1, ln, bpn, lpn, spn);
+        __builtin_amdgcn_s_setprio(0);
+        taken(c);
+        if (lane == lp) PutPivotRow<0>(a, sp, nv, rowC);
+        const float ldp = RlSel(ld, sp, lp);
+        RotSlots(a, ld, ldp, lc, nv, rowD, rowC);
+        storeCol(c + 1, col);
+        ++rot;
+        --nv;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) lc[i] = ln[i];
+        bp = bpn;
+        lp = lpn;
+        sp = spn;
+        ++c;
+    }
+
+    // The right-hand sides (the live slots left) to LDS by position; this wave's finished
+    // columns from original row order to final row order (all loads of a column complete
+    // before its stores).
+#pragma unroll
+    for (int kk = 0; kk < CPW; ++kk)
+    {
+        const int j = w + NW * (kk + rot);
+        if (kk < nv && j >= n && j < n + nrhs)
+#pragma unroll
+            for (int i = 0; i < RPL; ++i)
+                if (pos[i] != INT_MAX) L.y[j - n][pos[i]] = a[i][kk];
+    }
+    for (int j = w; j < n; j += NW)
+    {
+        float* col = A + static_cast<size_t>(j) * n;
+        float v[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) v[i] = pos[i] != INT_MAX ? col[lane + 64 * i] : 0.0f;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every lane's loads are done
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i)
+            if (pos[i] != INT_MAX) col[pos[i]] = v[i];
+    }
+    if (zero != 0 && lane == 0) atomicMin(&L.zmin, zero);
+    __syncthreads();
+    if (tid == 0) info[mi] = L.abort ? -1 : (L.zmin == INT_MAX ? 0 : L.zmin);
+    if (w != 0) return;
+
+    // Back substitution U x = y by one wave: lane l holds positions l + 64 s. Column k of U
+    // comes from the stored LU; x_k = y_k * (1 / U_kk); rows above k fold x_k in.
+    float y[kNrhsMax][RPL];
+#pragma unroll
+    for (int q = 0; q < kNrhsMax; ++q)
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) y[q][i] = (q < nrhs && lane + 64 * i < n) ? L.y[q][lane + 64 * i] : 0.0f;
+    BackSubSeg<RPL - 1, RPL>(A, n, lane, y);
+#pragma unroll
+    for (int q = 0; q < kNrhsMax; ++q)
+        if (q < nrhs)
+#pragma unroll
+            for (int i = 0; i < RPL; ++i)
+                if (lane + 64 * i < n) B[static_cast<size_t>(q) * n + lane + 64 * i] = y[q][i];
 }
 
 }  // namespace
@@ -426,8 +930,10 @@ struct VbatchedPlan
     int device = 0;
     int batch = 0;
     int nrhs = 0;
-    int numItems = 0;
-    int grid = 0;
+    int numItems = 0;  // items of the n <= 64 class
+    int grid = 0;      // persistent grid of the n <= 64 class
+    int first[3] = {0, 0, 0};  // 193..256, 129..192, 65..128: range in the order
+    int count[3] = {0, 0, 0};
     Item* dItems = nullptr;
     int* dOrder = nullptr;
     int* dN = nullptr;
@@ -449,9 +955,21 @@ static void FreePlan(VbatchedPlan* p)
 
 hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
 {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->dCounter);
-    hipLaunchKernelGGL(vbatched_kernel, dim3(p->grid), dim3(kWG), 0, s, p->dItems, p->numItems, p->dOrder, p->dN,
-                       p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo, p->dCounter);
+    if (p->count[0] > 0)
+        hipLaunchKernelGGL((vb_cyclic_kernel<4, 8>), dim3(p->count[0]), dim3(512), 0, s, p->dOrder + p->first[0], p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
+    if (p->count[1] > 0)
+        hipLaunchKernelGGL((vb_cyclic_kernel<3, 8>), dim3(p->count[1]), dim3(512), 0, s, p->dOrder + p->first[1], p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
+    if (p->count[2] > 0)
+        hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, s, p->dOrder + p->first[2], p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
+    if (p->numItems > 0)
+    {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->dCounter);
+        hipLaunchKernelGGL(vbatched_kernel, dim3(p->grid), dim3(kWG), 0, s, p->dItems, p->numItems, p->dOrder, p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo, p->dCounter);
+    }
     return hipGetLastError();
 }
 
@@ -544,25 +1062,22 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
     p->batch = batchSize;
     p->nrhs = nrhs;
     (void)hipGetDevice(&p->device);
-    // bins, largest first; within a bin, larger orders first (longest-processing-time order)
+    // classes, largest first; within a class, larger orders first (longest-processing-time
+    // order). The n <= 64 class is cut into items of up to 8 matrices.
     std::vector<int> order(batchSize);
     for (int i = 0; i < batchSize; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ns[x] > ns[y]; });
     std::vector<Item> items;
-    for (int i = 0; i < batchSize;)
     {
-        const int n = ns[order[i]];
-        const int bin = n <= 64 ? 0 : (n <= 128 ? 1 : 2);
-        const int per = bin == 0 ? 8 : (bin == 1 ? 4 : 1);
-        int cnt = 0;
-        while (cnt < per && i + cnt < batchSize)
+        const int lo[3] = {193, 129, 65};
+        int i = 0;
+        for (int c = 0; c < 3; ++c)
         {
-            const int m = ns[order[i + cnt]];
-            if ((m <= 64 ? 0 : (m <= 128 ? 1 : 2)) != bin) break;
-            ++cnt;
+            p->first[c] = i;
+            while (i < batchSize && ns[order[i]] >= lo[c]) ++i;
+            p->count[c] = i - p->first[c];
         }
-        items.push_back({bin, i, cnt});
-        i += cnt;
+        for (; i < batchSize; i += 8) items.push_back({i, std::min(8, batchSize - i)});
     }
     std::vector<long long> pivOff(batchSize + 1, 0);
     for (int i = 0; i < batchSize; ++i)
@@ -588,7 +1103,7 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
         if (e == hipSuccess) e = hipMemcpy(p->dN, ns.data(), batchSize * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(p->dPivOff, pivOff.data(), batchSize * sizeof(long long), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = Autotune(p, ns, s, H(handle)->numCUs);
+        if (e == hipSuccess && p->numItems > 0) e = Autotune(p, ns, s, H(handle)->numCUs);
     }
     if (e != hipSuccess)
     {
