@@ -564,22 +564,28 @@ __device__ __attribute__((always_inline)) inline void PivotColumn(float (&col)[R
                                                                   float (&l)[RPL], int& bpos, int& lp, int& sp,
                                                                   float& d)
 {
+    unsigned key[RPL];
     unsigned bkey = 0u;
-    int bp = INT_MAX, bi = 0;
-    float bv = 0.0f;
 #pragma unroll
     for (int i = 0; i < RPL; ++i)  // branch-free: bitwise logic and selects only
     {
         const bool cand = (pos[i] >= k) & (pos[i] != INT_MAX);
         const float v = col[i];
-        unsigned key = (__float_as_uint(v) & 0x7fffffffu) + 2u;
-        key = v != v ? 1u : key;
-        key = cand ? key : 0u;
-        const bool better = (key > bkey) | ((key == bkey) & (pos[i] < bp));
-        bkey = better ? key : bkey;
-        bp = better ? pos[i] : bp;
-        bi = better ? i : bi;
-        bv = better ? v : bv;
+        unsigned kv = (__float_as_uint(v) & 0x7fffffffu) + 2u;
+        kv = v != v ? 1u : kv;
+        key[i] = cand ? kv : 0u;
+        bkey = bkey > key[i] ? bkey : key[i];
+    }
+    int bp = INT_MAX;  // lowest position among this lane's rows holding bkey
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) bp = min(bp, key[i] == bkey ? pos[i] : INT_MAX);
+    int bi = 0;
+    float bv = col[0];
+#pragma unroll
+    for (int i = 1; i < RPL; ++i)
+    {
+        bi = pos[i] == bp ? i : bi;
+        bv = pos[i] == bp ? col[i] : bv;
     }
     const unsigned top = WaveReduceU32<true>(bkey);
     const unsigned long long hit = __builtin_amdgcn_ballot_w64(bkey == top);
@@ -746,18 +752,24 @@ __global__ __launch_bounds__(64 * NW) void vb_cyclic_kernel(const int* __restric
     {
         const int e = c % kRing;
         unsigned long long tg = 0ull;
+        float4 lv;
         for (int it = 0;; ++it)
         {
-            tg = __hip_atomic_load(&L.tag[e], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // The multipliers are read right behind the tag: LDS serves one wave's reads in
+            // order, so when the tag matches, the entry's data (written before the tag by
+            // its owner, behind a release fence) is what the second read returned.
+            tg = __hip_atomic_load(&L.tag[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);  // keep the data read behind the tag read
+            lv = L.l[e][lane];
             if (__builtin_amdgcn_readfirstlane(static_cast<unsigned>(tg)) == static_cast<unsigned>(c + 1)) break;
             if (Aborted(L.abort, it)) break;
             __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const unsigned meta = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(tg >> 32));
         bp = meta & 1023u;
         lp = (meta >> 10) & 63u;
         sp = meta >> 16;
-        const float4 lv = L.l[e][lane];
         lc[0] = lv.x;
         if constexpr (RPL > 1) lc[RPL > 1 ? 1 : 0] = lv.y;
         if constexpr (RPL > 2) lc[RPL > 2 ? 2 : 0] = lv.z;
