@@ -951,6 +951,8 @@ struct VbatchedPlan
     int* dN = nullptr;
     long long* dPivOff = nullptr;
     int* dCounter = nullptr;
+    hipStream_t side = nullptr;  // the n <= 128 classes run here, beside the larger ones
+    hipEvent_t fork = nullptr, join = nullptr;
     double flops = 0.0;
     double bytes = 0.0;
 };
@@ -962,27 +964,50 @@ static void FreePlan(VbatchedPlan* p)
     if (p->dN) (void)hipFree(p->dN);
     if (p->dPivOff) (void)hipFree(p->dPivOff);
     if (p->dCounter) (void)hipFree(p->dCounter);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    if (p->join) (void)hipEventDestroy(p->join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     delete p;
 }
 
+// The 193..256 and 129..192 classes run on the caller's stream; the 65..128 and n <= 64
+// classes run on the plan's side stream between a fork and a join event (a workgroup of the
+// 65..128 class fits in the registers and LDS that a 193..256 workgroup leaves free on a CU,
+// so it fills issue slots the latency-bound large class leaves idle). The pattern is
+// capturable into a hipGraph from the caller's stream.
 hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
 {
+    const bool small = p->count[2] > 0 || p->numItems > 0;
+    const bool big = p->count[0] > 0 || p->count[1] > 0;
+    hipStream_t t = (small && big) ? p->side : s;
+    if (t != s)
+    {
+        hipError_t e = hipEventRecord(p->fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(t, p->fork, 0);
+        if (e != hipSuccess) return e;
+    }
     if (p->count[0] > 0)
         hipLaunchKernelGGL((vb_cyclic_kernel<4, 8>), dim3(p->count[0]), dim3(512), 0, s, p->dOrder + p->first[0], p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
+    if (p->count[2] > 0)
+        hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, t, p->dOrder + p->first[2], p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
     if (p->count[1] > 0)
         hipLaunchKernelGGL((vb_cyclic_kernel<3, 8>), dim3(p->count[1]), dim3(512), 0, s, p->dOrder + p->first[1], p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
-    if (p->count[2] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, s, p->dOrder + p->first[2], p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
     if (p->numItems > 0)
     {
-        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, p->dCounter);
-        hipLaunchKernelGGL(vbatched_kernel, dim3(p->grid), dim3(kWG), 0, s, p->dItems, p->numItems, p->dOrder, p->dN,
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, t, p->dCounter);
+        hipLaunchKernelGGL(vbatched_kernel, dim3(p->grid), dim3(kWG), 0, t, p->dItems, p->numItems, p->dOrder, p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo, p->dCounter);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && t != s)
+    {
+        e = hipEventRecord(p->join, t);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, p->join, 0);
+    }
+    return e;
 }
 
 // Autotuner: times each candidate persistent-grid size once on synthetic diagonally
@@ -1109,6 +1134,9 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
         if (e == hipSuccess) e = hipMalloc(&p->dN, batchSize * sizeof(int));
         if (e == hipSuccess) e = hipMalloc(&p->dPivOff, batchSize * sizeof(long long));
         if (e == hipSuccess) e = hipMalloc(&p->dCounter, 256);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
         if (e == hipSuccess)
             e = hipMemcpy(p->dItems, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(p->dOrder, order.data(), batchSize * sizeof(int), hipMemcpyHostToDevice);
