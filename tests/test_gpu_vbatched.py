@@ -135,3 +135,47 @@ def test_vbatched_plan_graph_replay(torch):
         assert (info == 0).all()
     A.lib().aclsolverSgesvVbatched_destroyPlan(plan)
     h.close()
+
+
+@pytest.mark.parametrize("nrhs", [0, 1, 3])
+def test_vbatched_class_boundaries(torch, nrhs):
+    """Orders at the edges of the size classes (one wave per system up to 64, then the
+    column-cyclic classes 65..128, 129..192, 193..256), fewer than four right-hand sides,
+    and one singular system per class (info = its first zero pivot)."""
+    rng = np.random.default_rng(100 + nrhs)
+    sizes = [63, 64, 65, 66, 127, 128, 129, 130, 191, 192, 193, 194, 255, 256, 64, 128, 192, 256]
+    singular = (14, 15, 16, 17)
+    mats, rhs = _batch(rng, sizes, max(nrhs, 1), singular=singular)
+    if nrhs == 0:
+        rhs = [np.zeros((n, 1), np.float32) for n in sizes]  # a buffer the solve must leave alone
+    dA, dB, pa, pb = _upload(torch, mats, rhs)
+    b_before = [t.clone() for t in dB]
+    ipiv = torch.zeros(sum(sizes), dtype=torch.int32, device="cuda")
+    info = torch.full((len(sizes),), -9, dtype=torch.int32, device="cuda")
+    ns = np.array(sizes, np.int32)
+    with A.Handle() as h:
+        st = A.lib().aclsolverSgesvVbatched(h.h, len(sizes), ns.ctypes.data, nrhs, A.ptr(pa), A.ptr(pb),
+                                            A.ptr(ipiv), A.ptr(info))
+        torch.cuda.synchronize()
+    assert st == A.SUCCESS
+    hinfo = info.cpu().numpy()
+    hpiv = ipiv.cpu().numpy()
+    off = 0
+    for i, n in enumerate(sizes):
+        lu = A.from_device_cm(dA[i], n, n)
+        ref_lu, ref_piv, ref_info, margin = O.sgetrf(mats[i], margins=True)
+        piv = hpiv[off:off + n]
+        off += n
+        assert hinfo[i] == ref_info, (i, n, hinfo[i], ref_info)
+        if i in singular:
+            assert ref_info == n // 2 + 1
+            continue
+        bad = np.nonzero(margin < 1e-4)[0]
+        upto = int(bad[0]) if len(bad) else n
+        np.testing.assert_array_equal(piv[:upto], ref_piv[:upto])
+        assert O.lu_backward_error(mats[i], lu, piv) <= 10 * n * EPS32, (i, n)
+        if nrhs == 0:
+            assert torch.equal(dB[i], b_before[i]), "B must not be touched when nrhs = 0"
+        else:
+            x = A.from_device_cm(dB[i], n, nrhs)
+            assert O.solve_backward_error(mats[i], x, rhs[i]) <= 10 * n * EPS32, (i, n)
