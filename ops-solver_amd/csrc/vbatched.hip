@@ -7,7 +7,7 @@
 //                   grid pulling items from a device counter (longest first);
 //   65..128:        column-cyclic kernel, 4 waves, 2 rows per lane;
 //   129..192:       column-cyclic kernel, 8 waves, 3 rows per lane;
-//   193..256:       column-cyclic kernel, 8 waves, 4 rows per lane.
+//   193..256:       column-cyclic kernel, 16 waves, 4 rows per lane.
 // The classes are launched largest first, each class ordered by decreasing n
 // (longest-processing-time order).
 //
@@ -971,10 +971,10 @@ static void FreePlan(VbatchedPlan* p)
 }
 
 // The 193..256 and 129..192 classes run on the caller's stream; the 65..128 and n <= 64
-// classes run on the plan's side stream between a fork and a join event (a workgroup of the
-// 65..128 class fits in the registers and LDS that a 193..256 workgroup leaves free on a CU,
-// so it fills issue slots the latency-bound large class leaves idle). The pattern is
-// capturable into a hipGraph from the caller's stream.
+// classes run on the plan's side stream between a fork and a join event, so that they take
+// CUs as soon as the larger classes leave room (the 129..192 workgroups leave registers free)
+// instead of waiting for them to finish. The pattern is capturable into a hipGraph from the
+// caller's stream.
 hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
 {
     const bool small = p->count[2] > 0 || p->numItems > 0;
@@ -987,7 +987,7 @@ hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, i
         if (e != hipSuccess) return e;
     }
     if (p->count[0] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<4, 8>), dim3(p->count[0]), dim3(512), 0, s, p->dOrder + p->first[0], p->dN,
+        hipLaunchKernelGGL((vb_cyclic_kernel<4, 16>), dim3(p->count[0]), dim3(1024), 0, s, p->dOrder + p->first[0], p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
     if (p->count[2] > 0)
         hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, t, p->dOrder + p->first[2], p->dN,
