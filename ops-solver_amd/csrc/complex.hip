@@ -1124,12 +1124,15 @@ __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* co
     int pos = lane;
     bool used = false;
     int zero = 0;
+    // Slot j of S holds column (j + steps done) % NL: each step pivots slot 0 and rotates the
+    // slots down by one (the processed column moves to the last slot), so no slot is ever
+    // addressed by a run-time index, also where the step loop is not unrolled.
 #pragma unroll(NL <= 32 ? NL : 1)
     for (int k = 0; k < NL; ++k)
     {
         if (k < n)
         {
-            const float2 sk = GetCol(S, k);
+            const float2 sk = S[0];
             const uint32_t cand = (rowLive && !used) ? PivotScore(sk) : 0u;
             uint32_t best;
             int key;
@@ -1147,19 +1150,16 @@ __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* co
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const float2 d = s_row[g][k];
+            const float2 d = s_row[g][0];
             const bool z = d.x == 0.0f && d.y == 0.0f;
             if (z && zero == 0) zero = k + 1;
             const double2 r = Recip64(z ? make_float2(1.0f, 0.0f) : d);
             const float2 c = (isP || z) ? make_float2(0.0f, 0.0f) : Mul64(sk, r);
+            const float2 last = z ? sk : (isP ? make_float2(1.0f, 0.0f) : make_float2(-c.x, -c.y));
 #pragma unroll
-            for (int j = 0; j < NL; ++j)
-                if (j != k) S[j] = SubMul(S[j], c, s_row[g][j]);
-            if (!z)
-            {
-                SetCol(S, k, isP ? make_float2(1.0f, 0.0f) : make_float2(-c.x, -c.y));
-                if (isP) sigma = r;
-            }
+            for (int j = 1; j < NL; ++j) S[j - 1] = SubMul(S[j], c, s_row[g][j]);
+            S[NL - 1] = last;
+            if (!z && isP) sigma = r;
             if (isP) used = true;
             __builtin_amdgcn_wave_barrier();
         }
@@ -1170,8 +1170,11 @@ __global__ __launch_bounds__(256) void cmatinv_gj_kernel(int n, const float2* co
     if (rowLive)
     {
 #pragma unroll
-        for (int j = 0; j < NL; ++j)
-            if (j < n) C[static_cast<int64_t>(s_inv[g][j]) * ldc + pos] = Mul64(S[j], sigma);
+        for (int kk = 0; kk < NL; ++kk)
+        {
+            const int j = kk + n < NL ? kk + n : kk + n - NL;  // the column slot kk holds
+            if (j < n) C[static_cast<int64_t>(s_inv[g][j]) * ldc + pos] = Mul64(S[kk], sigma);
+        }
     }
     if (live && lane == 0) info[mat] = zero;
 }
