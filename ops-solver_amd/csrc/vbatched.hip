@@ -406,13 +406,16 @@ __global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ 
 // ---------------------------------------------------------------------------------------
 // Column-cyclic kernel for 65 <= n <= 256.
 
-constexpr int kRing = 16;  // LDS ring entries of published pivot columns
+// LDS ring entries of published pivot columns (a wave may run that many columns ahead of
+// the slowest one before it waits for an entry).
+template <int NW>
+constexpr int RingSize() { return NW >= 8 ? 32 : 16; }
 
 template <int RPL, int NW, int CPW>
 struct CycLds
 {
-    float4 l[kRing][64];            // multipliers of the published column, by lane (row slots .x..)
-    unsigned long long tag[kRing];  // (column index + 1) | pivot position, lane, slot << 32
+    float4 l[RingSize<NW>()][64];            // multipliers of the published column, by lane (row slots .x..)
+    unsigned long long tag[RingSize<NW>()];  // (column index + 1) | pivot position, lane, slot << 32
     int done[NW];                   // columns whose pivot information each wave has taken
     int zmin;                       // first zero pivot (1-based) over the owners
     int abort;                      // set when a wait exceeded its bound: every wave then runs out
@@ -682,6 +685,7 @@ __global__ __launch_bounds__(64 * NW) void vb_cyclic_kernel(const int* __restric
     constexpr int NMAX = 64 * RPL;
     constexpr int CPW = ((NMAX + kNrhsMax + NW - 1) / NW + 3) / 4 * 4;  // column slots per wave
     __shared__ CycLds<RPL, NW, CPW> L;
+    constexpr int kRing = RingSize<NW>();
     const int mi = order[blockIdx.x];
     const int n = nArr[mi];
     float* __restrict__ A = Aarr[mi];
