@@ -974,15 +974,14 @@ static void FreePlan(VbatchedPlan* p)
     delete p;
 }
 
-// The 193..256 and 129..192 classes run on the caller's stream; the 65..128 and n <= 64
-// classes run on the plan's side stream between a fork and a join event, so that they take
-// CUs as soon as the larger classes leave room (the 129..192 workgroups leave registers free)
-// instead of waiting for them to finish. The pattern is capturable into a hipGraph from the
-// caller's stream.
+// The 193..256 class runs on the caller's stream; the other classes run on the plan's side
+// stream between a fork and a join event, so that they take CUs as the large class's
+// workgroups drain (its tail) instead of waiting for the whole kernel. The pattern is
+// capturable into a hipGraph from the caller's stream.
 hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
 {
-    const bool small = p->count[2] > 0 || p->numItems > 0;
-    const bool big = p->count[0] > 0 || p->count[1] > 0;
+    const bool small = p->count[1] > 0 || p->count[2] > 0 || p->numItems > 0;
+    const bool big = p->count[0] > 0;
     hipStream_t t = (small && big) ? p->side : s;
     if (t != s)
     {
@@ -993,11 +992,11 @@ hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, i
     if (p->count[0] > 0)
         hipLaunchKernelGGL((vb_cyclic_kernel<4, 16>), dim3(p->count[0]), dim3(1024), 0, s, p->dOrder + p->first[0], p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
+    if (p->count[1] > 0)
+        hipLaunchKernelGGL((vb_cyclic_kernel<3, 8>), dim3(p->count[1]), dim3(512), 0, t, p->dOrder + p->first[1], p->dN,
+                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
     if (p->count[2] > 0)
         hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, t, p->dOrder + p->first[2], p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
-    if (p->count[1] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<3, 8>), dim3(p->count[1]), dim3(512), 0, s, p->dOrder + p->first[1], p->dN,
                            p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
     if (p->numItems > 0)
     {
