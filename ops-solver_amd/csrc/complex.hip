@@ -1195,10 +1195,10 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
     const bool rowLive = live && lane < n;
     const float2* A = live ? Aarr[mat] : nullptr;
     float2* C = live ? Carr[mat] : nullptr;
-    float2 LU[NL];
-#pragma unroll
-    for (int j = 0; j < NL; ++j)
-        LU[j] = (rowLive && j < n) ? A[static_cast<int64_t>(j) * lda + lane] : make_float2(0.0f, 0.0f);
+    // The factor entries of this lane's row are read from memory one column per step (the
+    // next column is fetched a step ahead), so no register array is indexed by the step.
+    auto factor = [&](int q) -> float2
+    { return (rowLive && q >= 0 && q < n) ? A[static_cast<int64_t>(q) * lda + lane] : make_float2(0.0f, 0.0f); };
     if (live && lane == 0)
     {
         for (int i = 0; i < n; ++i) s_perm[g][i] = i;
@@ -1217,11 +1217,14 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
     const int pr = rowLive ? s_perm[g][lane] : -1;
 #pragma unroll
     for (int j = 0; j < NL; ++j) X[j] = make_float2(j == pr ? 1.0f : 0.0f, 0.0f);
+    float2 fnext = factor(0);
 #pragma unroll(NL <= 32 ? NL : 1)
     for (int q = 0; q < NL; ++q)
     {
         if (q < n)
         {
+            const float2 fq = fnext;
+            fnext = factor(q + 1);
             if (lane == q)
             {
 #pragma unroll
@@ -1231,21 +1234,24 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (lane > q && rowLive)
             {
-                const float2 l = GetCol(LU, q);
+                const float2 l = fq;
 #pragma unroll
                 for (int j = 0; j < NL; ++j) X[j] = SubMul(X[j], l, s_row[g][j]);
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
+    fnext = factor(n - 1);
 #pragma unroll(NL <= 32 ? NL : 1)
     for (int q = NL - 1; q >= 0; --q)
     {
         if (q < n)
         {
+            const float2 fq = fnext;
+            fnext = factor(q - 1);
             if (lane == q)
             {
-                const float2 d = GetCol(LU, q);
+                const float2 d = fq;
                 const bool z = d.x == 0.0f && d.y == 0.0f;
                 const double2 r = Recip64(z ? make_float2(1.0f, 0.0f) : d);
 #pragma unroll
@@ -1259,17 +1265,14 @@ __global__ __launch_bounds__(256) void cgetri_small_kernel(int n, const float2* 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (lane < q && rowLive)
             {
-                const float2 uq = GetCol(LU, q);
+                const float2 uq = fq;
 #pragma unroll
                 for (int j = 0; j < NL; ++j) X[j] = SubMul(X[j], uq, s_row[g][j]);
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
-    float2 diag = make_float2(1.0f, 0.0f);
-#pragma unroll
-    for (int j = 0; j < NL; ++j)
-        if (j == lane) diag = LU[j];
+    const float2 diag = factor(lane);
     const unsigned long long zmask = __ballot(rowLive && diag.x == 0.0f && diag.y == 0.0f);
     const int shift = (g * NL) % 64;
     const unsigned long long gmask = (NL >= 64) ? ~0ull : ((~0ull >> (64 - NL)) << shift);
