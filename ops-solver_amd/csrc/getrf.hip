@@ -108,6 +108,39 @@ __device__ inline unsigned long long WaveMaxU64(unsigned long long v)
     return mx(mx(ReadLaneU64(v, 0), ReadLaneU64(v, 16)), mx(ReadLaneU64(v, 32), ReadLaneU64(v, 48)));
 }
 
+template <int CTRL>
+__device__ inline unsigned DppU32(unsigned v)
+{
+    return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline unsigned WaveMaxU32(unsigned v)
+{
+    auto mx = [](unsigned a, unsigned b) { return a > b ? a : b; };
+    v = mx(v, DppU32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, DppU32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, DppU32<0x124>(v));  // row_ror:4
+    v = mx(v, DppU32<0x128>(v));  // row_ror:8
+    const unsigned r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const unsigned r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return mx(mx(r0, r1), mx(r2, r3));
+}
+// The same maximum as WaveMaxU64, reduced word by word: the high words first (one 32-bit
+// DPP max per step), then, only when several lanes hold that high word, their low words.
+// Keys here hold a magnitude in the high word and a position in the low one, so the
+// second pass runs only on exact magnitude ties.
+__device__ inline unsigned long long WaveMaxU64Split(unsigned long long v)
+{
+    const unsigned hi = static_cast<unsigned>(v >> 32), lo = static_cast<unsigned>(v);
+    const unsigned mh = WaveMaxU32(hi);
+    const unsigned long long tie = __ballot(hi == mh);
+    unsigned ml;
+    if (__popcll(tie) == 1)  // uniform
+        ml = __builtin_amdgcn_readlane(lo, __builtin_ctzll(tie));
+    else
+        ml = WaveMaxU32(hi == mh ? lo : 0u);
+    return (static_cast<unsigned long long>(mh) << 32) | ml;
+}
+
 // ---------------------------------------------------------------------------------------
 // Leaf factorization: columns [c0, c0 + wact) of rows [c0, m), T threads, R rows per
 // thread (row tid + i*T), W >= wact columns in registers. Pivoting is implicit: each row
@@ -180,7 +213,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                 bi >= 0 ? (1ull << 63) | (static_cast<unsigned long long>(__float_as_uint(tmax) + 1u) << 32) |
                               static_cast<unsigned>(~tpos)
                         : 0ull;
-            const unsigned long long wmax = WaveMaxU64(best);
+            const unsigned long long wmax = WaveMaxU64Split(best);
             if (best == wmax && bi >= 0)  // the wave's winner (keys are unique per position)
             {
                 s_key[par][wid] = wmax;
@@ -408,7 +441,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 bi = k > best ? i : bi;
                 best = k > best ? k : best;
             }
-            const unsigned long long top = WaveMaxU64(best);
+            const unsigned long long top = WaveMaxU64Split(best);
             if (lane == 0)
             {
                 StoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
@@ -460,7 +493,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                     break;
                 }
             }
-            const unsigned long long gtop = WaveMaxU64(key0 > key1 ? key0 : key1);
+            const unsigned long long gtop = WaveMaxU64Split(key0 > key1 ? key0 : key1);
             const unsigned long long hit0 = __ballot(ok && lane < P && key0 == gtop);
             const unsigned long long hit1 = __ballot(ok && lane + 64 < P && key1 == gtop);
             const int winner = hit0 != 0ull   ? __ffsll(static_cast<long long>(hit0)) - 1
