@@ -270,27 +270,36 @@ constexpr size_t kCSlotBytes = 2ull * kCMaxGroups * kCSlotWords * sizeof(unsigne
 constexpr unsigned kCLeafSpin = 1u << 20;
 
 template <int CTRL>
-__device__ inline unsigned long long CDpp64(unsigned long long v)
+__device__ inline unsigned CDppU32(unsigned v)
 {
-    const int vlo = static_cast<int>(v & 0xffffffffull), vhi = static_cast<int>(v >> 32);
-    const int lo = __builtin_amdgcn_update_dpp(vlo, vlo, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(vhi, vhi, CTRL, 0xF, 0xF, false);
-    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
+    return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
 }
-__device__ inline unsigned long long CLane64(unsigned long long v, int l)
+__device__ inline unsigned CWaveMaxU32(unsigned v)
 {
-    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), l);
-    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), l);
-    return (static_cast<unsigned long long>(hi) << 32) | lo;
+    auto mx = [](unsigned a, unsigned b) { return a > b ? a : b; };
+    v = mx(v, CDppU32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, CDppU32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, CDppU32<0x124>(v));  // row_ror:4
+    v = mx(v, CDppU32<0x128>(v));  // row_ror:8
+    const unsigned r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const unsigned r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return mx(mx(r0, r1), mx(r2, r3));
 }
+// Wave-wide maximum of a 64-bit key, reduced word by word: the high words first (one 32-bit
+// DPP max per step), then, only when several lanes hold that high word, their low words
+// (the keys hold a pivot score in the high word and a position in the low one, so the
+// second pass runs only on exact score ties).
 __device__ inline unsigned long long CWaveMax64(unsigned long long v)
 {
-    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
-    v = mx(v, CDpp64<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = mx(v, CDpp64<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = mx(v, CDpp64<0x124>(v));  // row_ror:4
-    v = mx(v, CDpp64<0x128>(v));  // row_ror:8
-    return mx(mx(CLane64(v, 0), CLane64(v, 16)), mx(CLane64(v, 32), CLane64(v, 48)));
+    const unsigned hi = static_cast<unsigned>(v >> 32), lo = static_cast<unsigned>(v);
+    const unsigned mh = CWaveMaxU32(hi);
+    const unsigned long long tie = __ballot(hi == mh);
+    unsigned ml;
+    if (__popcll(tie) == 1)  // uniform
+        ml = __builtin_amdgcn_readlane(lo, __builtin_ctzll(tie));
+    else
+        ml = CWaveMaxU32(hi == mh ? lo : 0u);
+    return (static_cast<unsigned long long>(mh) << 32) | ml;
 }
 __device__ inline unsigned long long CLoadSlot(const unsigned long long* p)
 {
