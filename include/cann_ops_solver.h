@@ -16,11 +16,22 @@
  *    (reference :980-1043); the Device column-major solves are EXTENSIONS.
  *  - Work is asynchronous on the handle's stream; validation steps that inspect Device data
  *    synchronize that stream, as the reference does.
- *  - Size limits are extended beyond the reference's (getrf m <= 32768, matinv n <= 64,
- *    cgetrf m <= 16384); aclsolverSetStrictLimits(handle, 1) restores the reference limits.
+ *  - Size limits are extended beyond the reference's (getrf m <= 16384, getrs / getri
+ *    n <= 16384, matinv n <= 64, cgetrf m <= 16384); larger sizes get INVALID_VALUE with
+ *    devInfo = -(argument index). aclsolverSetStrictLimits(handle, 1) restores the
+ *    reference limits.
+ *  - Every compute entry point answers ACLSOLVER_STATUS_ARCH_MISMATCH when the handle was
+ *    created on a device other than gfx950 (the reference does so off its target,
+ *    src/utils/complex_solver_host.h:50-53).
+ *  - devInfo = ACLSOLVER_INFO_WAIT_TIMEOUT (EXTENSION) reports that a bounded
+ *    cross-workgroup wait of the factorization or solve gave up; the results are then NaN.
+ *    The call itself returns SUCCESS (the condition arises on the device after the launch).
  */
 #pragma once
 #include "cann_ops_solver_common.h"
+
+#define ACLSOLVER_INFO_WAIT_TIMEOUT (-1000)
+#define ACLSOLVER_DEFAULT_SPIN_LIMIT (1u << 20)
 
 #ifdef __cplusplus
 extern "C"
@@ -157,6 +168,14 @@ extern "C"
      * 2: uneven batched getrf+getrs. */
     aclsolverStatus_t aclsolverProfileBegin(void);
     aclsolverStatus_t aclsolverProfileEnd(int kernel, int* launches, double* work, double* ms);
+
+    /* ---- EXTENSION (testing): polls a bounded cross-workgroup wait may spend before it
+     * gives up (default ACLSOLVER_DEFAULT_SPIN_LIMIT; 0 makes every wait that is not
+     * already satisfied give up at once). Lowering it lets a test drive the timeout path
+     * from the host while every kernel still exits normally; results of calls that time
+     * out are NaN and devInfo reports ACLSOLVER_INFO_WAIT_TIMEOUT. Library-wide, not
+     * thread-safe against concurrent calls. */
+    aclsolverStatus_t aclsolverDebugSetSpinLimit(unsigned int polls);
 
 #ifdef __cplusplus
 }

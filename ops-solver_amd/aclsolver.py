@@ -20,7 +20,10 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cann_ops_solver.h")
 SUCCESS = 0
 INVALID_VALUE = 3
 EXECUTION_FAILED = 5
+ARCH_MISMATCH = 8
 HANDLE_IS_NULLPTR = 9
+INFO_WAIT_TIMEOUT = -1000
+DEFAULT_SPIN_LIMIT = 1 << 20
 ACL_ERROR_INVALID_PARAM = 100000
 
 _lib = None
@@ -83,6 +86,7 @@ def lib():
         "aclsolverSgesvVbatched": ([P, I, P, I, P, P, P, P], S),
         "aclsolverProfileBegin": ([], S),
         "aclsolverProfileEnd": ([I, P, P, P], S),
+        "aclsolverDebugSetSpinLimit": ([ctypes.c_uint], S),
     }
     for name, (args, res) in sig.items():
         if hasattr(L, name):

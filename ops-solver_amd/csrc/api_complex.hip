@@ -181,7 +181,7 @@ using namespace ops;
 OPS_API aclsolverStatus_t aclsolverCgetrf_bufferSize(aclsolverHandle_t handle, int m, int n, aclFloatComplex* A,
                                                      int lda, int* Lwork)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (m < 0 || m > RowLimit(handle) || n < 0 || n > ColLimit(handle) || lda < std::max(1, m) ||
         Lwork == nullptr || (m > 0 && n > 0 && !IsDevice(A)))
         return ACLSOLVER_STATUS_INVALID_VALUE;
@@ -192,7 +192,7 @@ OPS_API aclsolverStatus_t aclsolverCgetrf_bufferSize(aclsolverHandle_t handle, i
 OPS_API aclsolverStatus_t aclsolverCgetrf(aclsolverHandle_t handle, int m, int n, aclFloatComplex* A, int lda,
                                           aclFloatComplex* Workspace, int* devIpiv, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     int arg = 0;
     if (m < 0 || m > RowLimit(handle))
         arg = 1;
@@ -220,7 +220,7 @@ OPS_API aclsolverStatus_t aclsolverCgetrsDevice(aclsolverHandle_t handle, int tr
                                                 const aclFloatComplex* dA, int lda, const int* devIpiv,
                                                 aclFloatComplex* dB, int ldb, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     int arg = 0;
     if (trans < 0 || trans > 2)
         arg = 1;
@@ -318,7 +318,7 @@ static size_t CGetriWorkBytes(int n)
 OPS_API aclsolverStatus_t aclsolverCgetri_bufferSize(aclsolverHandle_t handle, int n, aclFloatComplex* A, int lda,
                                                      int* Lwork)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     const int lim = H(handle)->strict ? 4096 : 16384;
     if (n < 0 || n > lim || lda < std::max(1, n) || Lwork == nullptr || (n > 0 && A != nullptr && !IsDevice(A)))
         return ACLSOLVER_STATUS_INVALID_VALUE;
@@ -329,7 +329,7 @@ OPS_API aclsolverStatus_t aclsolverCgetri_bufferSize(aclsolverHandle_t handle, i
 OPS_API aclsolverStatus_t aclsolverCgetri(aclsolverHandle_t handle, int n, aclFloatComplex* A, int lda, int* devIpiv,
                                           aclFloatComplex* Workspace, int Lwork, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     const int lim = H(handle)->strict ? 4096 : 16384;
     int arg = 0;
     if (n < 0 || n > lim)
@@ -378,7 +378,7 @@ OPS_API aclsolverStatus_t aclsolverCgetriBatched(aclsolverHandle_t handle, int n
                                                  aclFloatComplex* const Carray[], int ldc, int* infoArray,
                                                  int batchSize)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     const int lim = H(handle)->strict ? 4096 : 16384;
     int arg = 0;
     if (n < 0 || n > lim)
@@ -427,7 +427,7 @@ OPS_API aclsolverStatus_t aclsolverCmatinvBatched(aclsolverHandle_t handle, int 
                                                   int lda, aclFloatComplex* const Ainv[], int lda_inv, int* info,
                                                   int batchSize)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     int arg = 0;
     if (n < 0 || n > MatinvLimit(handle))
         arg = 1;

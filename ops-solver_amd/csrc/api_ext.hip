@@ -22,7 +22,7 @@ constexpr int kGesvMax = 16384;
 
 aclsolverStatus_t GesvBufferSize(aclsolverHandle_t handle, int n, int nrhs, size_t* bytes)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (n < 0 || n > kGesvMax || nrhs < 0 || bytes == nullptr) return ACLSOLVER_STATUS_INVALID_VALUE;
     *bytes = n == 0 ? 0 : GesvIrWorkspaceBytes(n, H(handle)->irRestart);
     return ACLSOLVER_STATUS_SUCCESS;
@@ -33,7 +33,7 @@ aclsolverStatus_t GesvBufferSize(aclsolverHandle_t handle, int n, int nrhs, size
 aclsolverStatus_t Gesv(aclsolverHandle_t handle, Precision prec, int n, int nrhs, float* dA, int ldda, int* dipiv,
                        float* dB, int lddb, float* dX, int lddx, void* dWork, size_t lwork, int* niter, int* dinfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (dinfo == nullptr || !IsDevice(dinfo)) return ACLSOLVER_STATUS_INVALID_VALUE;
     Handle* h = H(handle);
     int arg = 0;

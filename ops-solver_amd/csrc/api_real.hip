@@ -25,8 +25,8 @@ namespace
 {
 constexpr int kMaxBatch = 10000000;  // src/utils/real_lu_common.h:18
 
-int FactorRowLimit(aclsolverHandle_t h) { return H(h)->strict ? 8192 : 32768; }
-int GetriLimit(aclsolverHandle_t h) { return H(h)->strict ? 4096 : 32768; }
+int FactorRowLimit(aclsolverHandle_t h) { return H(h)->strict ? 8192 : kEngineMaxOrder; }
+int GetriLimit(aclsolverHandle_t h) { return H(h)->strict ? 4096 : kEngineMaxOrder; }
 int MatinvLimit(aclsolverHandle_t h) { return H(h)->strict ? 32 : 64; }
 
 bool Overlap(const void* a, size_t aBytes, const void* b, size_t bBytes)
@@ -156,7 +156,7 @@ using namespace ops;
 OPS_API aclsolverStatus_t aclsolverSgetrf_bufferSize(aclsolverHandle_t handle, int m, int n, float* A, int lda,
                                                      int* Lwork)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (m < 0 || m > FactorRowLimit(handle) || n < 0 || lda < std::max(1, m) || Lwork == nullptr ||
         IsDevice(Lwork) || (A != nullptr && !IsDevice(A)))
         return ACLSOLVER_STATUS_INVALID_VALUE;
@@ -167,7 +167,7 @@ OPS_API aclsolverStatus_t aclsolverSgetrf_bufferSize(aclsolverHandle_t handle, i
 OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n, float* A, int lda, float* Workspace,
                                           int* devIpiv, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     // FactorArgument, src/sgetrf/sgetrf_host.cpp:46-73
     int arg = 0;
     if (m < 0 || m > FactorRowLimit(handle))
@@ -223,11 +223,11 @@ OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n
 OPS_API aclsolverStatus_t aclsolverSgetrsDevice(aclsolverHandle_t handle, int trans, int n, int nrhs, const float* dA,
                                                 int lda, const int* devIpiv, float* dB, int ldb, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     int arg = 0;
     if (trans < 0 || trans > 1)
         arg = 1;
-    else if (n < 0 || n > 32768)
+    else if (n < 0 || n > kEngineMaxOrder)
         arg = 2;
     else if (nrhs < 0)
         arg = 3;
@@ -249,7 +249,7 @@ OPS_API aclsolverStatus_t aclsolverSgetrsDevice(aclsolverHandle_t handle, int tr
     if (n == 0 || nrhs == 0) return Launched();
     void* work = HandleScratch(handle, RealGetrsWorkspaceBytes(n, nrhs));
     if (work == nullptr) return ACLSOLVER_STATUS_ALLOC_FAILED;
-    if (RealGetrs(trans, n, nrhs, dA, lda, devIpiv, dB, ldb, work, s) != hipSuccess)
+    if (RealGetrs(trans, n, nrhs, dA, lda, devIpiv, dB, ldb, work, s, devInfo) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }
@@ -261,7 +261,7 @@ OPS_API aclError aclsolverSgetrs(aclsolverHandle_t handle, const uint64_t trans,
                                  float* A, const int64_t lda, int32_t* ipiv, float* B, const int64_t ldb,
                                  int32_t* info)
 {
-    const int64_t lim = (handle != nullptr && H(handle)->strict) ? 8192 : 32768;
+    const int64_t lim = (handle != nullptr && H(handle)->strict) ? 8192 : kEngineMaxOrder;
     if (!(n > 0 && n <= lim && nrhs > 0 && nrhs <= lim && lda >= n && ldb >= nrhs && A != nullptr && B != nullptr &&
           ipiv != nullptr && info != nullptr && lda - n <= 16383 && ldb - nrhs <= 16383))
     {
@@ -285,7 +285,7 @@ OPS_API aclError aclsolverSgetrs(aclsolverHandle_t handle, const uint64_t trans,
     const int N = static_cast<int>(n), R = static_cast<int>(nrhs);
     const size_t aRow = static_cast<size_t>(n) * lda, bRow = static_cast<size_t>(n) * ldb;
     const size_t bytes = (aRow + static_cast<size_t>(N) * N + bRow + static_cast<size_t>(N) * R) * sizeof(float) +
-                         static_cast<size_t>(N) * sizeof(int) + RealGetrsWorkspaceBytes(N, R) + 1024;
+                         static_cast<size_t>(N + 1) * sizeof(int) + RealGetrsWorkspaceBytes(N, R) + 1024;
     char* base = nullptr;
     if (hipMalloc(&base, bytes) != hipSuccess)
     {
@@ -297,21 +297,30 @@ OPS_API aclError aclsolverSgetrs(aclsolverHandle_t handle, const uint64_t trans,
     float* dBrow = dA + static_cast<size_t>(N) * N;
     float* dB = dBrow + bRow;
     int* dPiv = reinterpret_cast<int*>(dB + static_cast<size_t>(N) * R);
-    void* work = reinterpret_cast<char*>(dPiv + N) + 256 - (reinterpret_cast<uintptr_t>(dPiv + N) % 256);
+    int* dInfo = dPiv + N;  // the solve's wait-timeout report
+    void* work = reinterpret_cast<char*>(dInfo + 1) + 256 - (reinterpret_cast<uintptr_t>(dInfo + 1) % 256);
+    FillInfo(dInfo, 1, 0, s);
     hipError_t e = hipMemcpyAsync(dArow, A, aRow * sizeof(float), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dBrow, B, bRow * sizeof(float), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dPiv, ipiv, N * sizeof(int), hipMemcpyHostToDevice, s);
     // Row-major (r, c) at X[r*ld + c] is column-major X^T: transpose into column-major.
     if (e == hipSuccess) e = TransposeRM(N, N, dArow, lda, dA, N, s);
     if (e == hipSuccess) e = TransposeRM(N, R, dBrow, ldb, dB, N, s);
-    if (e == hipSuccess) e = RealGetrs(static_cast<int>(trans), N, R, dA, N, dPiv, dB, N, work, s);
+    if (e == hipSuccess) e = RealGetrs(static_cast<int>(trans), N, R, dA, N, dPiv, dB, N, work, s, dInfo);
     // back: column-major (N x R, ld N) -> row-major with ldb: transpose of an R x N row-major view.
     if (e == hipSuccess) e = TransposeRM(R, N, dB, N, dBrow, ldb, s);
     if (e == hipSuccess) e = hipMemcpyAsync(B, dBrow, bRow * sizeof(float), hipMemcpyDeviceToHost, s);
+    int waitInfo = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&waitInfo, dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
     hipError_t sync = hipStreamSynchronize(s);
     if (sync != hipSuccess) return ACL_ERROR_RT_FAILURE;  // buffers retained: work may still be in flight
     (void)hipFree(base);
     if (e != hipSuccess) return ACL_ERROR_RT_FAILURE;
+    if (waitInfo != 0)
+    {
+        fprintf(stderr, "[ERROR] aclsolverSgetrs: a bounded wait of the triangular solve gave up.\n");
+        return ACL_ERROR_RT_FAILURE;
+    }
     *info = 0;
     return ACL_SUCCESS;
 }
@@ -326,7 +335,7 @@ static size_t GetriWorkBytes(int n)
 
 OPS_API aclsolverStatus_t aclsolverSgetri_bufferSize(aclsolverHandle_t handle, int n, float* A, int lda, int* Lwork)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (n < 0 || n > GetriLimit(handle) || lda < std::max(1, n) || Lwork == nullptr || IsDevice(Lwork) ||
         (A != nullptr && !IsDevice(A)))
         return ACLSOLVER_STATUS_INVALID_VALUE;
@@ -337,7 +346,7 @@ OPS_API aclsolverStatus_t aclsolverSgetri_bufferSize(aclsolverHandle_t handle, i
 OPS_API aclsolverStatus_t aclsolverSgetri(aclsolverHandle_t handle, int n, float* A, int lda, int* devIpiv,
                                           float* Workspace, int Lwork, int* devInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     int arg = 0;
     if (n < 0 || n > GetriLimit(handle))
         arg = 1;
@@ -389,7 +398,7 @@ OPS_API aclsolverStatus_t aclsolverSgetriBatched(aclsolverHandle_t handle, int n
                                                  const int* PivotArray, float* const Carray[], int ldc,
                                                  int* infoArray, int batchSize)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     // BatchedArgument (factor = false), src/utils/real_lu_common.h:40-60
     int arg = 0;
     if (n < 0 || n > GetriLimit(handle))
@@ -456,7 +465,7 @@ OPS_API aclsolverStatus_t aclsolverSgetriBatched(aclsolverHandle_t handle, int n
 OPS_API aclsolverStatus_t aclsolverSmatinvBatched(aclsolverHandle_t handle, int n, const float* const A[], int lda,
                                                   float* const Ainv[], int lda_inv, int* info, int batchSize)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     // BatchedArgument (factor = true): n=1, A=2, lda=3, Ainv=4, lda_inv=5, info=6, batch=7.
     int arg = 0;
     if (n < 0 || n > MatinvLimit(handle))
@@ -507,7 +516,7 @@ OPS_API aclsolverStatus_t aclsolverSmatinvStridedBatched(aclsolverHandle_t handl
                                                          long long strideA, float* Ainv, int lda_inv,
                                                          long long strideInv, int* ipiv, int* info, int batchSize)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (n < 0 || n > 64 || lda < std::max(1, n) || lda_inv < std::max(1, n) || batchSize < 0 ||
         strideA < static_cast<long long>(lda) * n || strideInv < static_cast<long long>(lda_inv) * n)
         return ACLSOLVER_STATUS_INVALID_VALUE;

@@ -32,12 +32,15 @@ namespace ops
 namespace
 {
 
-// Pivot key: larger |a| first, then the lower (position, lane); NaN never wins (|a| bits + 1
-// in the high word, 0 for NaN), non-candidates are 0.
-__device__ inline unsigned long long GjKey(float v, int posLane)
+// Pivot key: larger |a| first, then the lower (position, lane). NaN follows the
+// reference's scan, whose running best starts at the diagonal candidate and is replaced
+// only by a strictly larger magnitude (src/utils/kernel/real_lu_small_dispatch.hpp:270-286):
+// a NaN on the diagonal position wins (high word 0x7fffffff), a NaN anywhere else never
+// does (high word 0); numbers have |a| bits + 1. Non-candidates are 0.
+__device__ inline unsigned long long GjKey(float v, int posLane, bool diagonal)
 {
     const float a = fabsf(v);
-    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;
+    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : (diagonal ? 0x7fffffffu : 0u);
     return (1ull << 63) | (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~posLane);
 }
 
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(256) void matinv_gj_kernel(int n, const float* cons
     for (int k = 0; k < NL; ++k)
     {
         if (k >= n) continue;
-        const unsigned long long myKey = (rowLive && !used) ? GjKey(S[k], (pos << 8) | lane) : 0ull;
+        const unsigned long long myKey = (rowLive && !used) ? GjKey(S[k], (pos << 8) | lane, pos == k) : 0ull;
         const unsigned long long top = GroupMaxKey<NL>(myKey);
         // No candidate at all cannot happen for k < n (position k's row is a candidate).
         const int key = static_cast<int>(~static_cast<unsigned>(top & 0xffffffffull));

@@ -267,7 +267,6 @@ __global__ __launch_bounds__(128) void cpanel_swap_trsm_kernel(float2* __restric
 constexpr int kCSlotWords = 64;      // 2 key words + 2 words per complex column (W <= 16)
 constexpr int kCMaxGroups = 128;
 constexpr size_t kCSlotBytes = 2ull * kCMaxGroups * kCSlotWords * sizeof(unsigned long long);
-constexpr unsigned kCLeafSpin = 1u << 20;
 
 template <int CTRL>
 __device__ inline unsigned CDppU32(unsigned v)
@@ -316,7 +315,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                                                          int pivoting, int newPanel, int* __restrict__ ipiv,
                                                          int* __restrict__ info, int* __restrict__ rowsrc,
                                                          int* __restrict__ leafmap, unsigned long long* slots,
-                                                         unsigned tagBase, int* err)
+                                                         unsigned tagBase, int* err, unsigned spinLimit)
 {
     constexpr int T = 64;
     static_assert(2 * W + 2 <= kCSlotWords && 2 * W <= T, "slot layout");
@@ -390,7 +389,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             // the P keys: lane l reads workgroups l and l + 64
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
-            for (unsigned spins = 0; spins < kCLeafSpin; ++spins)
+            for (unsigned spins = 0; spins < spinLimit; ++spins)
             {
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
@@ -427,7 +426,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                 rw = lane < 2 * wact ? CLoadSlot(base + winner * kCSlotWords + 2 + lane)
                                      : static_cast<unsigned long long>(tag) << 32;
                 if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
-                if (spins >= kCLeafSpin) ok = false;
+                if (spins >= spinLimit) ok = false;
             }
             if (!ok)
             {
@@ -870,10 +869,18 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
         constexpr int R = 2, W = 16;
         const int groups = (mp + 64 * R - 1) / (64 * R);
         if (groups > kCMaxGroups || wact > W || w.epoch == nullptr) return hipErrorInvalidValue;
-        const unsigned tagBase = (*w.epoch)++ * 64u;
-        hipLaunchKernelGGL((cleaf_multi_kernel<R, W>), dim3(groups), dim3(64), 0, stream, A, lda, m, c0, wact,
-                           pivoting, newPanel, ipiv, info, w.rowsrc, w.leafmap, w.slots, tagBase, w.err);
-        return hipGetLastError();
+        unsigned tagBase = (*w.epoch)++ * 64u;
+        unsigned spinLimit = SpinLimit();
+        // Cooperative launch: the workgroups wait for each other every column, so all of
+        // them must be resident at once.
+        int* rowsrc = w.rowsrc;
+        int* leafmap = w.leafmap;
+        unsigned long long* slots = w.slots;
+        int* err = w.err;
+        void* args[] = {&A, &lda, &m, &c0, &wact, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &leafmap, &slots,
+                        &tagBase, &err, &spinLimit};
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W>), dim3(groups),
+                                          dim3(64), args, 0, stream);
     }
 #define CLEAF(T, R, W) LaunchCLeaf<T, R, W>(A, lda, m, j0, c0, wact, pivoting, newPanel, ipiv, info, w, stream)
     if (mp <= 256) return CLEAF(256, 1, 16);
@@ -962,6 +969,7 @@ hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, 
                               a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, stream));
         }
     }
+    if (m > kCLeafMultiRows) FoldWaitError(w.err, info, stream);
     return hipGetLastError();
 }
 

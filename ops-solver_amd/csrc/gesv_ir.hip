@@ -81,8 +81,12 @@ __global__ void gemv_reduce_kernel(int n, int chunks, const double* __restrict__
     out[i] = b ? b[i] - s : s;
 }
 
+// Maximum that keeps NaN (fmax drops it): a NaN anywhere in the vector makes the norm NaN,
+// so the refinement sees a non-finite backward error instead of a spurious zero.
+__device__ inline double NanMax(double a, double b) { return (a != a || b != b) ? __builtin_nan("") : fmax(a, b); }
+
 // Single-workgroup reductions into out[slot] (fixed tree order).
-template <int OP>  // 0: max |v|, 1: sum v^2
+template <int OP>  // 0: max |v| (NaN-propagating), 1: sum v^2
 __global__ __launch_bounds__(1024) void reduce_kernel(int n, const double* __restrict__ v, double* __restrict__ out,
                                                       int slot)
 {
@@ -91,13 +95,14 @@ __global__ __launch_bounds__(1024) void reduce_kernel(int n, const double* __res
     for (int i = threadIdx.x; i < n; i += 1024)
     {
         const double x = v[i];
-        acc = OP == 0 ? fmax(acc, fabs(x)) : fma(x, x, acc);
+        acc = OP == 0 ? NanMax(acc, fabs(x)) : fma(x, x, acc);
     }
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1)
     {
-        if (threadIdx.x < w) s[threadIdx.x] = OP == 0 ? fmax(s[threadIdx.x], s[threadIdx.x + w]) : s[threadIdx.x] + s[threadIdx.x + w];
+        if (threadIdx.x < w)
+            s[threadIdx.x] = OP == 0 ? NanMax(s[threadIdx.x], s[threadIdx.x + w]) : s[threadIdx.x] + s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) out[slot] = OP == 0 ? s[0] : sqrt(s[0]);
@@ -377,7 +382,20 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
                              useRbt ? "rbt" : "pivoted", outer, berr, totalIters, elapsed());
                 std::fflush(stderr);
             }
-            if (!(berr > tol)) break;
+            if (!std::isfinite(berr))
+            {
+                // Overflow in the factors (e.g. a tiny unpivoted pivot) or non-finite data:
+                // retry with partial pivoting once, otherwise report non-convergence.
+                if (useRbt)
+                {
+                    useRbt = false;
+                    if ((e = factor()) != hipSuccess) return e;
+                    goto restart_rhs;
+                }
+                converged = false;
+                break;
+            }
+            if (berr <= tol) break;
             if (outer++ >= maxOuter)
             {
                 if (useRbt)  // refinement stalled on the unpivoted factors: refactor with pivoting
@@ -470,6 +488,10 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
         }
         hipLaunchKernelGGL(f64_to_f32_kernel, G1(n), dim3(256), 0, stream, n, w.x, X + static_cast<size_t>(c) * ldx);
     }
+    // A bounded wait that gave up (factor leaves or chained solves) reaches dinfo too; the
+    // refinement loop has already turned it into an error for the refined precisions.
+    if (!useRbt) FoldWaitError(RealGetrfErrorWord(w.getrfWork, n), info, stream);
+    FoldWaitError(ts.err, info, stream);
     if (niterOut) *niterOut = converged ? totalIters : -totalIters - 1;
     return hipGetLastError();
 }

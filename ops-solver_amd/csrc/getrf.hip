@@ -80,35 +80,6 @@ inline Work Layout(void* base, int m)
 }
 
 template <int CTRL>
-__device__ inline unsigned long long DppU64(unsigned long long v)
-{
-    // Every control used here reads a valid lane, so the old value (v itself) is never kept.
-    const int vlo = static_cast<int>(v & 0xffffffffull), vhi = static_cast<int>(v >> 32);
-    const int lo = __builtin_amdgcn_update_dpp(vlo, vlo, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(vhi, vhi, CTRL, 0xF, 0xF, false);
-    return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo);
-}
-
-__device__ inline unsigned long long ReadLaneU64(unsigned long long v, int lane)
-{
-    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), lane);
-    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane);
-    return (static_cast<unsigned long long>(hi) << 32) | lo;
-}
-
-// Maximum over the wave64: quad swaps and row rotations (DPP) give every lane its 16-lane
-// row maximum, four lane reads combine the rows.
-__device__ inline unsigned long long WaveMaxU64(unsigned long long v)
-{
-    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
-    v = mx(v, DppU64<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = mx(v, DppU64<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = mx(v, DppU64<0x124>(v));  // row_ror:4
-    v = mx(v, DppU64<0x128>(v));  // row_ror:8
-    return mx(mx(ReadLaneU64(v, 0), ReadLaneU64(v, 16)), mx(ReadLaneU64(v, 32), ReadLaneU64(v, 48)));
-}
-
-template <int CTRL>
 __device__ inline unsigned DppU32(unsigned v)
 {
     return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
@@ -189,30 +160,24 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
         if (j < wact)  // uniform
         {
             const int par = j & 1;
-            // Candidate of this thread: its best row (or, without pivoting, the row at j).
-            // First the largest |a| over the candidate rows (fmaxf skips NaN, so a NaN
-            // never wins), then the lowest position among the rows that reach it; only the
-            // thread's winner builds a 64-bit key for the cross-thread argmax.
-            float tmax = -1.0f;
+            // Candidate of this thread: its best row (or, without pivoting, the row at j),
+            // as a 64-bit key (|a| bits + 1, then ~position, so the lowest position wins
+            // magnitude ties). NaN: on the diagonal position it is the pivot (0x7fffffff),
+            // elsewhere it never wins (0) -- src/utils/kernel/real_lu.hpp:59-77, :100-109.
+            unsigned long long best = 0ull;
+            int bi = -1;
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
                 const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
-                tmax = fmaxf(tmax, cand ? fabsf(a[i][j]) : -1.0f);
+                const unsigned ab = __float_as_uint(fabsf(a[i][j]));
+                const unsigned mag = ab <= 0x7f800000u ? ab + 1u : (pos[i] == j ? 0x7fffffffu : 0u);
+                const unsigned long long k =
+                    cand ? (1ull << 63) | (static_cast<unsigned long long>(mag) << 32) | static_cast<unsigned>(~pos[i])
+                         : 0ull;
+                bi = k > best ? i : bi;
+                best = k > best ? k : best;
             }
-            int tpos = INT_MAX, bi = -1;
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-            {
-                const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
-                const bool hit = cand && fabsf(a[i][j]) == tmax && pos[i] < tpos;
-                tpos = hit ? pos[i] : tpos;
-                bi = hit ? i : bi;
-            }
-            const unsigned long long best =
-                bi >= 0 ? (1ull << 63) | (static_cast<unsigned long long>(__float_as_uint(tmax) + 1u) << 32) |
-                              static_cast<unsigned>(~tpos)
-                        : 0ull;
             const unsigned long long wmax = WaveMaxU64Split(best);
             if (best == wmax && bi >= 0)  // the wave's winner (keys are unique per position)
             {
@@ -354,13 +319,11 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 // updates still come in pivot order, so the result equals the unblocked elimination.
 // Slots alternate with the column parity: a workgroup publishes column j+2 only after all
 // have published j+1, i.e. after all have finished reading column j. Pivot choice, scaling
-// and update order are those of leaf_getf2_kernel (a NaN candidate ranks below every
-// number; if all candidates are NaN the row at position j is the pivot). Spins are bounded:
+// and update order are those of leaf_getf2_kernel (a NaN on the diagonal position is the
+// pivot, a NaN elsewhere ranks below every number). Spins are bounded:
 // on a timeout the workgroup raises *err, fills its rows with NaN and leaves the column
 // loop; its peers then time out in turn.
 // ---------------------------------------------------------------------------------------
-constexpr unsigned kLeafSpin = 1u << 20;
-
 __device__ inline unsigned long long LoadSlot(const unsigned long long* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -377,7 +340,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                                                         int panelEnd, int pivoting, int newPanel,
                                                         int* __restrict__ ipiv, int* __restrict__ info,
                                                         int* __restrict__ rowsrc, unsigned long long* slots,
-                                                        unsigned tagBase, int* err)
+                                                        unsigned tagBase, int* err, unsigned spinLimit)
 {
     constexpr int T = 64;
     static_assert(W + 2 <= kSlotWords && W <= T, "slot layout");
@@ -433,10 +396,9 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             {
                 const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
                 const unsigned ab = __float_as_uint(fabsf(a[i][j]));
+                const unsigned mag = ab <= 0x7f800000u ? ab + 1u : (pos[i] == j ? 0x7fffffffu : 0u);
                 const unsigned long long k =
-                    cand ? (1ull << 63) |
-                               (static_cast<unsigned long long>(ab <= 0x7f800000u ? ab + 1u : 0u) << 32) |
-                               static_cast<unsigned>(~pos[i])
+                    cand ? (1ull << 63) | (static_cast<unsigned long long>(mag) << 32) | static_cast<unsigned>(~pos[i])
                          : 0ull;
                 bi = k > best ? i : bi;
                 best = k > best ? k : best;
@@ -468,7 +430,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             // The P keys: lane l reads the two key words of workgroups l and l + 64.
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
-            for (unsigned spins = 0; spins < kLeafSpin; ++spins)
+            for (unsigned spins = 0; spins < spinLimit; ++spins)
             {
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
@@ -506,7 +468,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 rw = lane < wact ? LoadSlot(base + winner * kSlotWords + 2 + lane)
                                  : static_cast<unsigned long long>(tag) << 32;
                 if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
-                if (spins >= kLeafSpin) ok = false;
+                if (spins >= spinLimit) ok = false;
             }
             if (!ok)
             {
@@ -761,10 +723,17 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
     const int mp = m - c0;
     const int groups = (mp + kMultiT * kMultiR - 1) / (kMultiT * kMultiR);
     if (groups > kMaxLeafGroups || wact > kMultiW || w.epoch == nullptr) return hipErrorInvalidValue;
-    const unsigned tagBase = (*w.epoch)++ * 64u;
-    hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW>), dim3(groups), dim3(kMultiT), 0, stream, A, lda,
-                       m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w.rowsrc, w.slots, tagBase, w.err);
-    return hipGetLastError();
+    unsigned tagBase = (*w.epoch)++ * 64u;
+    unsigned spinLimit = SpinLimit();
+    // Cooperative launch: the workgroups wait for each other every column, so the runtime
+    // must make all of them resident at once (it refuses a grid that cannot be).
+    int* rowsrc = w.rowsrc;
+    unsigned long long* slots = w.slots;
+    int* err = w.err;
+    void* args[] = {&A, &lda, &m, &j0, &c0, &wact, &panelEnd, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &slots,
+                    &tagBase, &err, &spinLimit};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW>),
+                                      dim3(groups), dim3(kMultiT), args, 0, stream);
 }
 
 // Leaf width for a panel of mp active rows: every shape keeps R*W <= 128 values per lane
@@ -936,6 +905,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         }
     }
     if (two) RET_IF(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
+    if (UseMultiLeaf(m)) FoldWaitError(w.err, info, s0);
     return hipGetLastError();
 }
 
@@ -1106,7 +1076,7 @@ hipError_t TransposeRM(int rows, int cols, const float* src, int64_t lds, float*
 }
 
 hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const int* ipiv, float* b, int ldb,
-                     void* work, hipStream_t stream)
+                     void* work, hipStream_t stream, int* info)
 {
     if (n <= 0 || nrhs <= 0) return hipSuccess;
     float* x = static_cast<float*>(work);
@@ -1122,6 +1092,7 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
             RET_IF(TrsvChainReset(ts, stream));
             RET_IF(TrsvChain(ts, false, lu, lda, x, stream));
             RET_IF(TrsvChain(ts, true, lu, lda, x, stream));
+            FoldWaitError(ts.err, info, stream);
         }
         else
             RET_IF(RealLuSolveInPlace(0, n, nrhs, lu, lda, x, n, stream));

@@ -42,6 +42,7 @@ struct Handle
     int numCUs = 256;
     LookAhead la;             // created on first use
     bool laReady = false;
+    bool archOk = true;       // the current device at creation is a gfx950
 };
 
 // The handle's look-ahead resources (created on first call, default stream priority);
@@ -50,6 +51,28 @@ const LookAhead* HandleLookAhead(aclsolverHandle_t h);
 
 inline Handle* H(aclsolverHandle_t h) { return static_cast<Handle*>(h); }
 inline hipStream_t Stream(aclsolverHandle_t h) { return H(h)->stream; }
+
+// Entry-point guard: HANDLE_IS_NULLPTR for a null handle, ARCH_MISMATCH when the handle
+// was created on a device other than gfx950, the only target the kernels are built for
+// (reference src/utils/complex_solver_host.h:50-53 answers ARCH_MISMATCH off its target).
+inline aclsolverStatus_t CheckHandle(aclsolverHandle_t h)
+{
+    if (h == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    return H(h)->archOk ? ACLSOLVER_STATUS_SUCCESS : ACLSOLVER_STATUS_ARCH_MISMATCH;
+}
+
+// Largest order the register-resident factorization engines support (rows of getrf,
+// order of getrs / getri / gesv).
+constexpr int kEngineMaxOrder = 16384;
+
+// Polls a bounded wait may spend before it gives up (multi-workgroup panel leaves,
+// chained triangular solves). Library-wide; lowered only to test the failure path.
+unsigned SpinLimit();
+// devInfo value a factorization or solve reports when one of its bounded waits gave up
+// (its results are then NaN); ACLSOLVER_INFO_WAIT_TIMEOUT in the public header.
+constexpr int kInfoWaitTimeout = -1000;
+// info <- kInfoWaitTimeout when *err != 0 (one thread, stream-ordered after the waits).
+void FoldWaitError(const int* err, int* info, hipStream_t stream);
 
 // True when pointer is Device memory of the current process (hipMalloc / managed).
 bool IsDevice(const void* p);
@@ -120,8 +143,9 @@ hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void
                      hipStream_t stream);
 // Device column-major getrs from RealGetrf factors (trans 0 / 1). work: RealGetrsWorkspaceBytes.
 size_t RealGetrsWorkspaceBytes(int n, int nrhs);
+// info (optional, Device): set to kInfoWaitTimeout when a bounded wait of the solve gave up.
 hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const int* ipiv, float* b, int ldb,
-                     void* work, hipStream_t stream);
+                     void* work, hipStream_t stream, int* info = nullptr);
 
 // ---- complex LU engine (complex_engine.hip) ----
 size_t ComplexGetrfWorkspaceBytes(int m, int n);

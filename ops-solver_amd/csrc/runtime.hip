@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <cstring>
 
 #include <new>
 
@@ -12,6 +13,24 @@
 
 namespace ops
 {
+
+namespace
+{
+unsigned g_spinLimit = ACLSOLVER_DEFAULT_SPIN_LIMIT;
+
+__global__ void fold_wait_error_kernel(const int* err, int* info)
+{
+    if (*err != 0) *info = kInfoWaitTimeout;
+}
+}  // namespace
+
+unsigned SpinLimit() { return g_spinLimit; }
+
+void FoldWaitError(const int* err, int* info, hipStream_t stream)
+{
+    if (err == nullptr || info == nullptr) return;
+    hipLaunchKernelGGL(fold_wait_error_kernel, dim3(1), dim3(1), 0, stream, err, info);
+}
 
 bool IsDevice(const void* p)
 {
@@ -184,7 +203,15 @@ OPS_API aclsolverStatus_t aclsolverCreate(aclsolverHandle_t* handle)
         return ACLSOLVER_STATUS_ALLOC_FAILED;
     }
     hipDeviceProp_t prop{};
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->numCUs = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    {
+        h->numCUs = prop.multiProcessorCount;
+        // gcnArchName is e.g. "gfx950:sramecc+:xnack-". OPS_SOLVER_ARCH_OVERRIDE replaces the
+        // reported name (testing the mismatch path on a gfx950 device).
+        const char* over = std::getenv("OPS_SOLVER_ARCH_OVERRIDE");
+        const char* arch = (over != nullptr && over[0] != 0) ? over : prop.gcnArchName;
+        h->archOk = std::strncmp(arch, "gfx950", 6) == 0 && (arch[6] == 0 || arch[6] == ':');
+    }
     *handle = h;
     return ACLSOLVER_STATUS_SUCCESS;
 }
@@ -277,5 +304,11 @@ OPS_API aclsolverStatus_t aclsolverSetRefinement(aclsolverHandle_t handle, doubl
     H(handle)->irTolerance = tolerance;
     H(handle)->irMaxOuter = maxOuter;
     H(handle)->irRestart = restart;
+    return ACLSOLVER_STATUS_SUCCESS;
+}
+
+OPS_API aclsolverStatus_t aclsolverDebugSetSpinLimit(unsigned int polls)
+{
+    g_spinLimit = polls;
     return ACLSOLVER_STATUS_SUCCESS;
 }

@@ -23,13 +23,13 @@
 
 #include <cstdlib>
 
+#include "internal.h"
 #include "trsv.h"
 
 namespace ops
 {
 namespace
 {
-constexpr unsigned kSpinLimit = 1u << 20;  // ~1 s of polling at ~1 us per poll
 
 __device__ inline unsigned long long PollWord(const unsigned long long* p)
 {
@@ -63,7 +63,7 @@ template <int NB, bool UPPER, typename V>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const float* __restrict__ T, int ldt,
                                                          V* __restrict__ x, unsigned long long* __restrict__ pub,
                                                          unsigned* __restrict__ ticket, unsigned base, unsigned epoch,
-                                                         int* __restrict__ err)
+                                                         int* __restrict__ err, unsigned spinLimit)
 {
     constexpr int RPL = NB / 64;  // rows per lane
     constexpr int CPW = NB / 4;   // tile columns per wave
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const float* __r
 #pragma unroll
             for (int h = 0; h < NW; ++h) ok = ok && static_cast<unsigned>(w[h] >> 32) == epoch;
             if (__all(ok)) break;
-            if (++spins > kSpinLimit)
+            if (++spins > spinLimit)
             {
                 failed = true;
                 break;
@@ -307,9 +307,10 @@ hipError_t LaunchChain(TrsvChainState& s, bool upper, const float* T, int ldt, V
     }
     const unsigned epoch = static_cast<unsigned>(s.launches) + 1u;
     const unsigned base = static_cast<unsigned>(s.launches) * static_cast<unsigned>(nblk);
+    const unsigned spinLimit = SpinLimit();
 #define CHAIN(NB, UP)                                                                                             \
     hipLaunchKernelGGL((trsv_chain_kernel<NB, UP, V>), dim3(nblk), dim3(256), 0, stream, n, T, ldt, x, s.pub, s.ticket, \
-                       base, epoch, s.err)
+                       base, epoch, s.err, spinLimit)
     if (nb == 128)
     {
         if (upper)

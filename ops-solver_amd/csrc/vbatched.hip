@@ -14,7 +14,7 @@
 // Column-cyclic kernel (one matrix per workgroup): lane l of every wave holds rows
 // l, l + 64, ... of the matrix; wave w holds columns w, w + NW, w + 2 NW, ... of the augmented
 // matrix [A | B]. The pivot search of column k runs entirely inside its owner wave (a 64-bit
-// key argmax: largest |a|, lowest position on ties, NaN never wins); the owner publishes the
+// key argmax: largest |a|, lowest position on ties, a NaN wins only on the diagonal); the owner publishes the
 // pivot position and the multipliers of its rows through a ring of LDS entries guarded by
 // per-entry flags, and every wave updates its own columns with the pivot row read by
 // v_readlane from its own registers. No workgroup barrier runs during the elimination: a
@@ -51,10 +51,13 @@ struct Item
     int count;  // matrices in this item (<= 8)
 };
 
-__device__ inline unsigned long long KeyOf(float v, int pos)
+// NaN on the diagonal position (pos == k) wins, NaN elsewhere never does: the reference's
+// running best starts at the diagonal and only a strictly larger |a| replaces it
+// (src/utils/kernel/real_lu.hpp:59-77, :100-109).
+__device__ inline unsigned long long KeyOf(float v, int pos, int k)
 {
     const float a = fabsf(v);
-    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : 0u;  // NaN never wins
+    const unsigned hi = (a == a) ? __float_as_uint(a) + 1u : (pos == k ? 0x7fffffffu : 0u);
     return (1ull << 63) | (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned>(~pos);
 }
 
@@ -166,7 +169,7 @@ __device__ __attribute__((always_inline)) void solve_group(int n, int nLoop, int
         const bool ownK = (k % TPR) == h;
         // (1) candidates of column k: the wave's best key, then one slot per wave
         const unsigned long long myKey =
-            (active && ownK && pos >= k && pos != INT_MAX) ? KeyOf(cur, pos) : 0ull;
+            (active && ownK && pos >= k && pos != INT_MAX) ? KeyOf(cur, pos, k) : 0ull;
         const unsigned long long wmax = WaveMax64(myKey);
         if (myKey != 0ull && myKey == wmax)
         {
@@ -556,9 +559,10 @@ __device__ inline unsigned WaveReduceU32(unsigned v)
 }
 
 // Pivot of column k from the owner's values col (rows by slot): among the rows not yet
-// pivoted, the largest |a| and, on ties, the lowest position (NaN never beats a number; a
-// column of zeros and NaNs takes its lowest position). A 32-bit key orders the magnitudes
-// (0: not a candidate, 1: NaN, else bits(|a|) + 2) and is reduced across the wave in one
+// pivoted, the largest |a| and, on ties, the lowest position; a NaN on the diagonal
+// position k is the pivot, a NaN elsewhere never wins (src/utils/kernel/real_lu.hpp:100-109).
+// A 32-bit key orders the magnitudes (0: not a candidate, 1: NaN below the diagonal,
+// 0xffffffff: NaN on the diagonal, else bits(|a|) + 2) and is reduced across the wave in one
 // pass; ties across lanes are rare and resolved by a second (min position) pass. Then the
 // multipliers of the rows below the pivot after the interchange (reciprocal scaling when
 // |d| >= FLT_MIN, division otherwise) replace their values in col.
@@ -575,7 +579,7 @@ __device__ __attribute__((always_inline)) inline void PivotColumn(float (&col)[R
         const bool cand = (pos[i] >= k) & (pos[i] != INT_MAX);
         const float v = col[i];
         unsigned kv = (__float_as_uint(v) & 0x7fffffffu) + 2u;
-        kv = v != v ? 1u : kv;
+        kv = v != v ? (pos[i] == k ? 0xffffffffu : 1u) : kv;
         key[i] = cand ? kv : 0u;
         bkey = bkey > key[i] ? bkey : key[i];
     }
@@ -1091,7 +1095,7 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
                                                             const int* nArray, int nrhs,
                                                             aclsolverVbatchedPlan_t* plan)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (plan == nullptr || batchSize < 0 || (batchSize > 0 && nArray == nullptr) || nrhs < 0 || nrhs > kNrhsMax)
         return ACLSOLVER_STATUS_INVALID_VALUE;
     std::vector<int> ns(nArray, nArray + batchSize);
@@ -1162,7 +1166,7 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_execute(aclsolverHandle_t handl
                                                          float* const dAarray[], float* const dBarray[], int* dIpiv,
                                                          int* dInfo)
 {
-    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (plan == nullptr) return ACLSOLVER_STATUS_INVALID_VALUE;
     VbatchedPlan* p = static_cast<VbatchedPlan*>(plan);
     if (p->batch == 0) return ACLSOLVER_STATUS_SUCCESS;

@@ -187,6 +187,40 @@ def test_getrs_matches_lapack_all_trans(n):
         np.testing.assert_allclose(x, ref, rtol=0, atol=1e-4 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n", [1, 3, 64, 257])
+def test_dgetrs_matches_lapack(n):
+    """orc_dgetrs (the FP64 reference of every device getrs comparison) against SciPy's
+    LAPACK dgetrs in float64, trans 0 and 1, on pivots from the oracle's own dgetrf and on a
+    hand-made pivot sequence with repeated and self interchanges."""
+    rng = np.random.default_rng(900 + n)
+    a = rng.standard_normal((n, n)) / np.sqrt(n)
+    lu, piv, info = O.dgetrf(a)
+    assert info == 0
+    b = rng.standard_normal((n, 4))
+    for t in (0, 1):
+        x = O.dgetrs(lu, piv, b, trans=t)
+        ref = sla.lu_solve((lu, piv - 1), b, trans=t)
+        np.testing.assert_allclose(x, ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+        # and the solve actually inverts op(A)
+        opa = a if t == 0 else a.T
+        assert np.abs(opa @ x - b).max() <= 1e-12 * n * max(1.0, np.abs(x).max())
+    # a planted transposition mistake (trans 0 result used for trans 1) must fail the pin
+    if n >= 3:
+        wrong = O.dgetrs(lu, piv, b, trans=0)
+        ref1 = sla.lu_solve((lu, piv - 1), b, trans=1)
+        assert np.abs(wrong - ref1).max() > 1e-6
+    # pivot sequences with repeats (ipiv[k] = n for every k) and identity entries
+    if n >= 2:
+        piv2 = np.full(n, n, dtype=np.int32)
+        piv2[-1] = n
+        lu2 = np.tril(rng.standard_normal((n, n)), -1) * 0.1 + np.triu(rng.standard_normal((n, n))) * 0.1
+        lu2[np.diag_indices(n)] = 1.0 + np.abs(lu2[np.diag_indices(n)])
+        for t in (0, 1):
+            x = O.dgetrs(lu2, piv2, b, trans=t)
+            ref = sla.lu_solve((lu2, piv2 - 1), b, trans=t)
+            np.testing.assert_allclose(x, ref, rtol=0, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
 def test_inverse_closed_forms():
     # docs/zh/smatinv_batched.md example: diag(2, 4) -> diag(0.5, 0.25), exact.
     c, piv, info = O.smatinv(np.diag([2.0, 4.0]).astype(np.float32))
