@@ -97,6 +97,19 @@ extern "C"
                                             const aclFloatComplex* dA, int lda, const int* devIpiv,
                                             aclFloatComplex* dB, int ldb, int* devInfo);
 
+    /* ---- EXTENSION without a counterpart in the reference: a solve that allocates nothing ----
+     * For callers that cannot allocate device memory (code under stream/graph capture, tight
+     * serving loops, harnesses that forbid hipMalloc): the same computation and results as
+     * aclsolverSgetrsDevice, with its staging (4*n*nrhs + 20*n bytes plus alignment) in a
+     * fixed per-GPU device array of ACLSOLVER_NOALLOC_SCRATCH_BYTES instead of handle scratch.
+     * Larger problems return INVALID_VALUE with devInfo = -2 (n alone too large) or -3
+     * (n*nrhs too large). Every call on one GPU shares the array: calls that overlap on one
+     * GPU (two streams or threads) race on it. */
+#define ACLSOLVER_NOALLOC_SCRATCH_BYTES (8u << 20)
+    aclsolverStatus_t aclsolverSgetrsDeviceNoAlloc(aclsolverHandle_t handle, int trans, int n, int nrhs,
+                                                   const float* dA, int lda, const int* devIpiv, float* dB, int ldb,
+                                                   int* devInfo);
+
     /* ---- EXTENSION: mixed-precision dense solve with GMRES iterative refinement ----
      * Argument convention of the reference's aclsolverSSgels (include/cann_ops_solver.h:
      * 815-862, the cuSOLVER IRS form). A is FP32 column-major and is NOT modified; the

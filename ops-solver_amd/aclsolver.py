@@ -24,6 +24,7 @@ ARCH_MISMATCH = 8
 HANDLE_IS_NULLPTR = 9
 INFO_WAIT_TIMEOUT = -1000
 DEFAULT_SPIN_LIMIT = 1 << 20
+NOALLOC_SCRATCH_BYTES = 8 << 20
 ACL_ERROR_INVALID_PARAM = 100000
 
 _lib = None
@@ -66,6 +67,7 @@ def lib():
         "aclsolverSmatinvStridedBatched": ([P, I, P, I, ctypes.c_longlong, P, I, ctypes.c_longlong, P, P, I], S),
         "aclsolverSgetrs": ([P, U64, I64, I64, P, I64, P, P, I64, P], I),
         "aclsolverSgetrsDevice": ([P, I, I, I, P, I, P, P, I, P], S),
+        "aclsolverSgetrsDeviceNoAlloc": ([P, I, I, I, P, I, P, P, I, P], S),
         "aclsolverCgetrf_bufferSize": ([P, I, I, P, I, P], S),
         "aclsolverCgetrf": ([P, I, I, P, I, P, P, P], S),
         "aclsolverCgetri_bufferSize": ([P, I, P, I, P], S),

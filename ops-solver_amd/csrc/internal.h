@@ -83,6 +83,10 @@ bool IsHostOnly(const void* p);
 // only touched by work on the handle's stream, so stream order protects reuse.
 void* HandleScratch(aclsolverHandle_t h, size_t bytes);
 
+// Device address of this device's fixed no-allocation scratch array (noalloc.hip), looked
+// up once per device with the device current; nullptr when the lookup fails.
+void* NoAllocScratch(int device);
+
 // Writes value into count ints of a Device info array on stream.
 void FillInfo(int* info, int count, int value, hipStream_t stream);
 // Invalid argument: writes -argument into info (when it is a Device pointer) and returns

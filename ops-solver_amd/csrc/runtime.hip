@@ -212,6 +212,9 @@ OPS_API aclsolverStatus_t aclsolverCreate(aclsolverHandle_t* handle)
         const char* arch = (over != nullptr && over[0] != 0) ? over : prop.gcnArchName;
         h->archOk = std::strncmp(arch, "gfx950", 6) == 0 && (arch[6] == 0 || arch[6] == ':');
     }
+    // Device address of the no-allocation entry point's scratch array, looked up once per
+    // device while it is current (failure here only disables that entry point).
+    if (h->archOk) (void)NoAllocScratch(device);
     *handle = h;
     return ACLSOLVER_STATUS_SUCCESS;
 }
