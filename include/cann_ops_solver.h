@@ -97,6 +97,16 @@ extern "C"
                                             const aclFloatComplex* dA, int lda, const int* devIpiv,
                                             aclFloatComplex* dB, int ldb, int* devInfo);
 
+    /* ---- EXTENSION: LU with tournament pivoting (communication-avoiding LU) ----
+     * Same arguments, workspace (aclsolverSgetrf_bufferSize), checks and outputs as
+     * aclsolverSgetrf, but panels with more than 1024 active rows choose the pivots of each
+     * 32 columns by a reduction tree over 256-row blocks (partial pivoting inside every
+     * block and tree node) instead of partial pivoting over the whole column. P A = L U
+     * holds with ipiv as the interchange sequence; the row order differs from partial
+     * pivoting's, |L| may exceed 1 slightly. A null devIpiv factors without pivoting. */
+    aclsolverStatus_t aclsolverSgetrfTournament(aclsolverHandle_t handle, int m, int n, float* A, int lda,
+                                                float* Workspace, int* devIpiv, int* devInfo);
+
     /* ---- EXTENSION without a counterpart in the reference: a solve that allocates nothing ----
      * For callers that cannot allocate device memory (code under stream/graph capture, tight
      * serving loops, harnesses that forbid hipMalloc): the same computation and results as
@@ -117,7 +127,8 @@ extern "C"
      * FP16 (SH: each FP32 operand split into two FP16 parts after a power-of-two row /
      * column scaling, three products) or BF16 (SB: split into three BF16 parts, six
      * products, see aclsolverSetBf16Split) inputs and FP32 accumulation; SS factors in plain
-     * FP32, all with partial pivoting (see aclsolverSetButterfly for the unpivoted form).
+     * FP32 with partial pivoting; SH/SB pivot partially too unless tournament pivoting is
+     * selected (aclsolverSetTournamentPivoting; aclsolverSetButterfly for the unpivoted form).
      * Refinement:
      * GMRES preconditioned by the LU factors (applied by FP64 substitution), the
      * residual r = b - A x and the correction accumulation in FP64, until the backward
@@ -149,6 +160,11 @@ extern "C"
      * butterfly transforms (n % 4 == 0; partial pivoting after a zero pivot or a refinement
      * failure); 0 (default) = partial pivoting. */
     aclsolverStatus_t aclsolverSetButterfly(aclsolverHandle_t handle, int on);
+    /* EXTENSION: pivoting of SH/SB's tall panel leaves. 1: tournament pivoting
+     * (communication-avoiding LU: the pivots of 32 columns are chosen by a reduction tree over
+     * 256-row blocks, so the row order differs from partial pivoting's); 0 (default): partial
+     * pivoting, as sgetrf. SS always uses partial pivoting. */
+    aclsolverStatus_t aclsolverSetTournamentPivoting(aclsolverHandle_t handle, int on);
 
     /* ---- EXTENSION: uneven batched LU solve (getrf + getrs) in one persistent launch ----
      * nArray (HOST, batchSize entries) holds each order n_i; dAarray / dBarray are Device

@@ -68,6 +68,8 @@ def lib():
         "aclsolverSgetrs": ([P, U64, I64, I64, P, I64, P, P, I64, P], I),
         "aclsolverSgetrsDevice": ([P, I, I, I, P, I, P, P, I, P], S),
         "aclsolverSgetrsDeviceNoAlloc": ([P, I, I, I, P, I, P, P, I, P], S),
+        "aclsolverSetTournamentPivoting": ([P, I], S),
+        "aclsolverSgetrfTournament": ([P, I, I, P, I, P, P, P], S),
         "aclsolverCgetrf_bufferSize": ([P, I, I, P, I, P], S),
         "aclsolverCgetrf": ([P, I, I, P, I, P, P, P], S),
         "aclsolverCgetri_bufferSize": ([P, I, P, I, P], S),

@@ -10,7 +10,8 @@ namespace ops
 {
 size_t GesvIrWorkspaceBytes(int n, int restart);
 hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
-                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, int* info,
+                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, bool tournament,
+                  int* info,
                   int* niterOut, hipStream_t stream, const LookAhead* la);
 }  // namespace ops
 
@@ -76,7 +77,8 @@ aclsolverStatus_t Gesv(aclsolverHandle_t handle, Precision prec, int n, int nrhs
         piv = static_cast<int*>(extra);
     }
     if (GesvIR(n, nrhs, dA, ldda, piv, dB, lddb, dX, lddx, dWork, prec, h->irTolerance, h->irMaxOuter, h->irRestart,
-               h->butterfly != 0, dinfo, niter, s, HandleLookAhead(handle)) != hipSuccess)
+               h->butterfly != 0, prec != Precision::FP32 && h->tournament != 0, dinfo, niter, s,
+               HandleLookAhead(handle)) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }
@@ -129,6 +131,15 @@ OPS_API aclsolverStatus_t aclsolverSetButterfly(aclsolverHandle_t handle, int on
 {
     if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
     H(handle)->butterfly = on != 0;
+    return ACLSOLVER_STATUS_SUCCESS;
+}
+// EXTENSION: pivoting of SH/SB's tall panel leaves. 1: tournament pivoting (the pivots of 32
+// columns chosen by a reduction tree over 256-row blocks, no exchange between CUs per
+// column); 0 (default): partial pivoting, as sgetrf. SS always pivots partially.
+OPS_API aclsolverStatus_t aclsolverSetTournamentPivoting(aclsolverHandle_t handle, int on)
+{
+    if (handle == nullptr) return ACLSOLVER_STATUS_HANDLE_IS_NULLPTR;
+    H(handle)->tournament = on != 0;
     return ACLSOLVER_STATUS_SUCCESS;
 }
 OPS_API aclsolverStatus_t aclsolverSSgesv(aclsolverHandle_t handle, int n, int nrhs, float* dA, int ldda, int* dipiv,

@@ -12,8 +12,6 @@
 
 namespace ops
 {
-hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la);
 hipError_t MatinvBatched(int n, const float* const* Aarr, const float* Abase, int lda, int64_t strideA,
                          float* const* Carr, float* Cbase, int ldc, int64_t strideC, int* ipiv, int* info, int batch,
                          hipStream_t stream);
@@ -164,8 +162,10 @@ OPS_API aclsolverStatus_t aclsolverSgetrf_bufferSize(aclsolverHandle_t handle, i
     return ACLSOLVER_STATUS_SUCCESS;
 }
 
-OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n, float* A, int lda, float* Workspace,
-                                          int* devIpiv, int* devInfo)
+namespace
+{
+aclsolverStatus_t Sgetrf(aclsolverHandle_t handle, int m, int n, float* A, int lda, float* Workspace, int* devIpiv,
+                         int* devInfo, bool tournament)
 {
     if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     // FactorArgument, src/sgetrf/sgetrf_host.cpp:46-73
@@ -212,9 +212,23 @@ OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n
     if (!AllFinite(handle, A, m, n, lda, 0, 1, &finite)) return ACLSOLVER_STATUS_EXECUTION_FAILED;
     if (!finite) return Invalid(handle, devInfo, 1, 3);
     if (RealGetrfPerm(m, n, A, lda, devIpiv, devInfo, Workspace, Precision::FP32, nullptr, s, nullptr,
-                      HandleLookAhead(handle)) != hipSuccess)
+                      HandleLookAhead(handle), tournament) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
+}
+}  // namespace
+
+OPS_API aclsolverStatus_t aclsolverSgetrf(aclsolverHandle_t handle, int m, int n, float* A, int lda, float* Workspace,
+                                          int* devIpiv, int* devInfo)
+{
+    return Sgetrf(handle, m, n, A, lda, Workspace, devIpiv, devInfo, false);
+}
+
+// EXTENSION: the same factorization with tournament pivoting on the tall panels.
+OPS_API aclsolverStatus_t aclsolverSgetrfTournament(aclsolverHandle_t handle, int m, int n, float* A, int lda,
+                                                    float* Workspace, int* devIpiv, int* devInfo)
+{
+    return Sgetrf(handle, m, n, A, lda, Workspace, devIpiv, devInfo, true);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -31,8 +31,6 @@
 
 namespace ops
 {
-hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la);
 hipError_t RealLuSolveInPlace(int trans, int n, int nrhs, const float* lu, int lda, float* x, int ldx,
                               hipStream_t stream);
 hipError_t RbtMakeDiagonals(int n, unsigned long long seed, float* du, float* dv, hipStream_t s);
@@ -258,7 +256,8 @@ size_t GesvIrWorkspaceBytes(int n, int restart) { return IrLayout(nullptr, n, re
 // Returns hipSuccess; *niterOut = total GMRES iterations (negative when refinement did not
 // reach tol within maxOuter outer steps).
 hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const float* B, int ldb, float* X, int ldx,
-                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, int* info,
+                  void* work, Precision prec, double tol, int maxOuter, int restart, bool butterfly, bool tournament,
+                  int* info,
                   int* niterOut, hipStream_t stream, const LookAhead* la)
 {
     IrWork w = IrLayout(work, n, restart);
@@ -284,7 +283,7 @@ hipError_t GesvIR(int n, int nrhs, const float* A, int lda, int* ipiv, const flo
             return err == hipSuccess ? hipGetLastError() : err;
         }
         hipLaunchKernelGGL(copy_cm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, stream, n, n, A, lda, w.lu, n);
-        return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream, w.pack, la);
+        return RealGetrfPerm(n, n, w.lu, n, ipiv, info, w.getrfWork, prec, w.perm, stream, w.pack, la, tournament);
     };
     TrsvChainState ts = TrsvChainLayout(w.trsv, n);
     if ((e = TrsvChainReset(ts, stream)) != hipSuccess) return e;

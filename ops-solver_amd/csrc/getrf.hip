@@ -27,6 +27,8 @@
 
 #include "lu_kernels.h"
 #include "trsv.h"
+#include "tslu.h"
+#include "wave.h"
 
 namespace ops
 {
@@ -57,6 +59,8 @@ struct Work
     unsigned* epoch;       // host counter: tag base of the next multi-workgroup leaf
     float* linvNode;       // block inverses for the panel-internal solves (panel stream only)
     float* linvPanel[2];   // block inverses of panel k's L11 in slot k % 2 (both streams read them)
+    void* tslu;            // scratch of the tournament leaves
+    bool tournament;       // tall leaves pivot by tournament (TsluLeaf)
 };
 inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
 inline Work Layout(void* base, int m)
@@ -76,40 +80,9 @@ inline Work Layout(void* base, int m)
     w.linvNode = reinterpret_cast<float*>(q);
     w.linvPanel[0] = reinterpret_cast<float*>(q + kLinvPanelBytes);
     w.linvPanel[1] = reinterpret_cast<float*>(q + 2 * kLinvPanelBytes);
+    w.tslu = reinterpret_cast<void*>(q + 3 * kLinvPanelBytes);
+    w.tournament = false;
     return w;
-}
-
-template <int CTRL>
-__device__ inline unsigned DppU32(unsigned v)
-{
-    return static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
-}
-__device__ inline unsigned WaveMaxU32(unsigned v)
-{
-    auto mx = [](unsigned a, unsigned b) { return a > b ? a : b; };
-    v = mx(v, DppU32<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = mx(v, DppU32<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = mx(v, DppU32<0x124>(v));  // row_ror:4
-    v = mx(v, DppU32<0x128>(v));  // row_ror:8
-    const unsigned r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
-    const unsigned r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
-    return mx(mx(r0, r1), mx(r2, r3));
-}
-// The same maximum as WaveMaxU64, reduced word by word: the high words first (one 32-bit
-// DPP max per step), then, only when several lanes hold that high word, their low words.
-// Keys here hold a magnitude in the high word and a position in the low one, so the
-// second pass runs only on exact magnitude ties.
-__device__ inline unsigned long long WaveMaxU64Split(unsigned long long v)
-{
-    const unsigned hi = static_cast<unsigned>(v >> 32), lo = static_cast<unsigned>(v);
-    const unsigned mh = WaveMaxU32(hi);
-    const unsigned long long tie = __ballot(hi == mh);
-    unsigned ml;
-    if (__popcll(tie) == 1)  // uniform
-        ml = __builtin_amdgcn_readlane(lo, __builtin_ctzll(tie));
-    else
-        ml = WaveMaxU32(hi == mh ? lo : 0u);
-    return (static_cast<unsigned long long>(mh) << 32) | ml;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -736,10 +709,24 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
                                       dim3(groups), dim3(kMultiT), args, 0, stream);
 }
 
+// Panels with more active rows than this take tournament leaves when the caller allows
+// them; OPS_SOLVER_TSLU_ROWS overrides it.
+int TsluRows()
+{
+    static const int rows = []
+    {
+        const char* e = std::getenv("OPS_SOLVER_TSLU_ROWS");
+        return e != nullptr ? std::atoi(e) : 1024;
+    }();
+    return rows;
+}
+inline bool UseTslu(const Work& w, int mp) { return w.tournament && mp > TsluRows() && mp <= kTsluMaxRows; }
+
 // Leaf width for a panel of mp active rows: every shape keeps R*W <= 128 values per lane
 // (512-thread workgroups leave up to 256 VGPRs per lane).
-int LeafWidth(int mp)
+int LeafWidth(int mp, const Work& w)
 {
+    if (UseTslu(w, mp)) return kTsluWidth;
     if (UseMultiLeaf(mp)) return kMultiW;
     if (mp <= 512) return 32;
     if (mp <= 1024) return 16;
@@ -751,6 +738,11 @@ hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd
                 int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
     const int mp = m - c0;
+    // Algorithmic traffic of a leaf: its panel columns read and written once.
+    ProfileScope prof(ProfileKernel::PanelLeaf, 8.0 * mp * wact, stream);
+    if (pivoting && UseTslu(w, mp))
+        return TsluLeaf(A, lda, m, j0, c0, wact, panelEnd, newPanel, ipiv, info, w.rowsrc, w.tslu, (*w.epoch)++,
+                        w.err, stream);
     if (UseMultiLeaf(mp)) return MultiLeaf(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream);
 #define LEAF(T, R, W) \
     LaunchLeaf<T, R, W>(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream)
@@ -779,7 +771,7 @@ hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd
 hipError_t PanelRec(float* a, int lda, int m, int j0, int panelEnd, int c0, int w, int pivoting, int* ipiv, int* info,
                     const Work& wk, hipStream_t stream)
 {
-    const int W = LeafWidth(m - c0);
+    const int W = LeafWidth(m - c0, wk);
     if (w <= W) return Leaf(a, lda, m, j0, c0, w, panelEnd, pivoting, c0 == j0, ipiv, info, wk, stream);
     const int h = (w / 2 + W - 1) / W * W;  // whole leaves on the left
     RET_IF(PanelRec(a, lda, m, j0, panelEnd, c0, h, pivoting, ipiv, info, wk, stream));
@@ -812,27 +804,33 @@ int OuterBlock(int m, int n, Precision prec)
 size_t RealGetrfWorkspaceBytes(int m, int n)
 {
     (void)n;
-    return WorkInts(std::max(m, 1)) * sizeof(int) + 8 + kSlotBytes + 16 + 256 + 3 * kLinvPanelBytes;
+    // The tournament leaves' scratch only where one can run (a panel taller than TsluRows).
+    return WorkInts(std::max(m, 1)) * sizeof(int) + 8 + kSlotBytes + 16 + 256 + 3 * kLinvPanelBytes +
+           (m > TsluRows() ? TsluScratchBytes(m) : 0);
 }
 
-int* RealGetrfErrorWord(void* work, int m) { return UseMultiLeaf(m) ? Layout(work, std::max(m, 1)).err : nullptr; }
+int* RealGetrfErrorWord(void* work, int m) { return Layout(work, std::max(m, 1)).err; }
 
 // ipiv == nullptr selects no pivoting. perm (optional, m ints) receives the row
 // permutation P with (P A)[i] = A[perm[i]]. pack (optional, BF16 / BF16X3 trailing
 // precisions): GemmPackedWorkspaceBytes(m, n, 512) bytes for the packed trailing update.
 hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
-                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la)
+                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la, bool tournament)
 {
     if (m > kMaxRows) return hipErrorInvalidValue;
     Work w = Layout(work, m);
     if (permOut) w.perm = permOut;
+    w.tournament = tournament && ipiv != nullptr && m > TsluRows();
+    if (w.tournament) RET_IF(TsluClear(w.tslu, m, stream));
     unsigned epoch = 1;
     w.epoch = &epoch;
     FillInfo(info, 1, 0, stream);
-    if (UseMultiLeaf(m))
     {
-        constexpr int words = static_cast<int>(kSlotBytes / sizeof(unsigned long long));
-        hipLaunchKernelGGL(clear_slots_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, w.slots, words, w.err);
+        // The multi-workgroup leaves' exchange slots (when they run) and the timeout flag
+        // shared by every bounded wait of the factorization.
+        const int words = UseMultiLeaf(m) ? static_cast<int>(kSlotBytes / sizeof(unsigned long long)) : 0;
+        hipLaunchKernelGGL(clear_slots_kernel, dim3(std::max(1, (words + 255) / 256)), dim3(256), 0, stream, w.slots,
+                           words, w.err);
     }
     const int kmax = std::min(m, n);
     if (kmax == 0) return hipGetLastError();
@@ -905,7 +903,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         }
     }
     if (two) RET_IF(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
-    if (UseMultiLeaf(m)) FoldWaitError(w.err, info, s0);
+    FoldWaitError(w.err, info, s0);
     return hipGetLastError();
 }
 

@@ -35,6 +35,7 @@ struct Handle
     int irRestart = 30;
     int bf16Split = 2;        // SBgesv trailing updates: 0 plain bf16, 1 two-part, 2 three-part split
     int butterfly = 0;        // SH/SB: random butterflies + no pivoting instead of partial pivoting
+    int tournament = 0;       // SH/SB: tall panel leaves pivot by tournament (0: partial pivoting)
     void* scratch = nullptr;  // handle-owned Device scratch (grown on demand)
     int* dflag = nullptr;     // Device flag word for validation scans
     int* hflag = nullptr;     // pinned host mirror of dflag
@@ -106,7 +107,8 @@ enum class ProfileKernel
 {
     TrailingGemm = 0,  // the packed BF16/FP16 MFMA trailing-update kernel alone (2 M N K flops)
     BatchedInverse = 1,
-    Vbatched = 2
+    Vbatched = 2,
+    PanelLeaf = 3  // one panel leaf of the LU (all its launches); work = 8 * rows * columns bytes
 };
 bool ProfilingOn();
 // Brackets the launches issued during its lifetime with HIP events on `stream`.
@@ -137,14 +139,22 @@ enum class Precision
 };
 // Workspace bytes needed by RealGetrf for an m x n matrix.
 size_t RealGetrfWorkspaceBytes(int m, int n);
-// Device address of the factorization's timeout flag inside its workspace (nonzero after a
-// multi-workgroup leaf gave up waiting for its peers; the factors then hold NaN), or
-// nullptr when a factorization of m rows runs no such leaf.
+// Device address of the factorization's timeout flag inside its workspace: nonzero after a
+// bounded wait of a panel leaf gave up (a multi-workgroup leaf waiting for its peers, or a
+// tournament node for its children's candidates); the factors are then invalid.
 int* RealGetrfErrorWord(void* work, int m);
 // Blocked right-looking LU with partial pivoting (or none when ipiv == nullptr).
 // Writes ipiv (1-based) and info (first zero pivot). Returns hipSuccess on launch success.
 hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
                      hipStream_t stream);
+// The blocked LU with look-ahead behind RealGetrf. permOut (optional, m ints) receives the
+// row permutation P with (P A)[i] = A[perm[i]]; pack (optional): the packed-operand buffer
+// of the mixed-precision trailing updates; la (optional): the side stream and events of the
+// look-ahead. tournament: tall panel leaves choose their pivots by tournament (a reduction
+// tree over row blocks) instead of partial pivoting -- for callers that only need stable
+// factors, not the partial-pivoting row order.
+hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, void* work, Precision trailing,
+                         int* permOut, hipStream_t stream, void* pack, const LookAhead* la, bool tournament = false);
 // Device column-major getrs from RealGetrf factors (trans 0 / 1). work: RealGetrsWorkspaceBytes.
 size_t RealGetrsWorkspaceBytes(int n, int nrhs);
 // info (optional, Device): set to kInfoWaitTimeout when a bounded wait of the solve gave up.

@@ -122,6 +122,92 @@ ORC_EXPORT int orc_sgetrf(int m, int n, float *a, int lda, int *ipiv, int divide
     return info;
 }
 
+/* Tournament pivot selection of one panel leaf (no reference counterpart: the library's
+ * communication-avoiding factorization, aclsolverSgetrfTournament, and the SH/SB solves).
+ * Rows [0, m) x columns [0, w) of a. Level 0 splits the rows into blocks of node_rows; every
+ * node factors its rows (in stacking order) with orc_sgetrf's blocked rule and keeps the
+ * rows of its first min(w, rows) pivots, in pivot order, with their ORIGINAL values; a node
+ * of the next level stacks the kept rows of `group` consecutive children. The root's
+ * pivot order, turned into a LAPACK interchange sequence, is written to ipiv[0, w)
+ * (1-based, relative to row 0). Returns the root's info. A matrix of at most node_rows
+ * rows is a single node: the result is orc_sgetrf's. */
+ORC_EXPORT int orc_stslu_pivots(int m, int w, const float *a, int lda, int node_rows, int group, int *ipiv)
+{
+    int nodes = (m + node_rows - 1) / node_rows;
+    int *ids = (int *)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    int *cnt = (int *)malloc(sizeof(int) * (size_t)(nodes > 0 ? nodes : 1));
+    int *next = (int *)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    int *ncnt = (int *)malloc(sizeof(int) * (size_t)(nodes > 0 ? nodes : 1));
+    float *blk = (float *)malloc(sizeof(float) * (size_t)node_rows * (size_t)w);
+    int *piv = (int *)malloc(sizeof(int) * (size_t)w);
+    int *order = (int *)malloc(sizeof(int) * (size_t)node_rows);
+    int info = 0;
+    /* level 0: node b holds rows [b*node_rows, ...) */
+    for (int b = 0; b < nodes; ++b)
+    {
+        int r0 = b * node_rows, nr = m - r0 < node_rows ? m - r0 : node_rows;
+        for (int i = 0; i < nr; ++i) ids[(size_t)b * node_rows + i] = r0 + i;
+        cnt[b] = nr;
+    }
+    for (;;)
+    {
+        /* factor every node; keep its first min(w, rows) pivot rows */
+        for (int b = 0; b < nodes; ++b)
+        {
+            int nr = cnt[b];
+            const int *id = ids + (size_t)b * node_rows;
+            for (int j = 0; j < w; ++j)
+                for (int i = 0; i < nr; ++i) blk[(size_t)j * nr + i] = a[(size_t)j * lda + id[i]];
+            int steps = nr < w ? nr : w;
+            int nodeinfo = orc_sgetrf(nr, w, blk, nr, piv, 0, NULL);
+            for (int i = 0; i < nr; ++i) order[i] = id[i];
+            for (int k = 0; k < steps; ++k)
+            {
+                int p = piv[k] - 1, t = order[k];
+                order[k] = order[p];
+                order[p] = t;
+            }
+            if (nodes == 1)
+            {
+                info = nodeinfo;
+                /* root: interchange sequence of its pivot rows (row ids are panel rows) */
+                int *cpos = (int *)malloc(sizeof(int) * (size_t)w);
+                for (int l = 0; l < steps; ++l) cpos[l] = order[l];
+                for (int j = 0; j < steps; ++j)
+                {
+                    int x = cpos[j];
+                    ipiv[j] = x + 1;
+                    for (int l = j + 1; l < steps; ++l)
+                        if (cpos[l] == j) cpos[l] = x;
+                }
+                free(cpos);
+                goto done;
+            }
+            for (int k = 0; k < steps; ++k) next[(size_t)b * w + k] = order[k];
+            ncnt[b] = steps;
+        }
+        /* stack the kept rows of `group` children per parent */
+        int parents = (nodes + group - 1) / group;
+        for (int q = 0; q < parents; ++q)
+        {
+            int nr = 0;
+            for (int c = q * group; c < nodes && c < (q + 1) * group; ++c)
+                for (int k = 0; k < ncnt[c]; ++k) ids[(size_t)q * node_rows + nr++] = next[(size_t)c * w + k];
+            cnt[q] = nr;
+        }
+        nodes = parents;
+    }
+done:
+    free(ids);
+    free(cnt);
+    free(next);
+    free(ncnt);
+    free(blk);
+    free(piv);
+    free(order);
+    return info;
+}
+
 /* Row interchanges of a column-major block b (n rows, nrhs columns), forward order. */
 static void laswp_f(float *b, int ldb, int nrhs, const int *ipiv, int k0, int k1, int forward)
 {

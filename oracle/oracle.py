@@ -31,6 +31,8 @@ def lib():
         i = ctypes.c_int
         _lib.orc_sgetrf.argtypes = [i, i, p, i, p, i, p]
         _lib.orc_sgetrf.restype = i
+        _lib.orc_stslu_pivots.argtypes = [i, i, p, i, i, i, p]
+        _lib.orc_stslu_pivots.restype = i
         _lib.orc_sgetrs.argtypes = [i, i, i, p, i, p, p, i]
         _lib.orc_sgetri.argtypes = [i, p, i, p]
         _lib.orc_sgetri.restype = i
@@ -73,6 +75,16 @@ def sgetrf(a, pivot=True, divide=False, margins=False):
     if margins:
         return lu, ipiv, info, margin[:k]
     return lu, ipiv, info
+
+
+def stslu_pivots(a, w, node_rows=256, group=8):
+    """Tournament pivots of the leaf a[:, :w] (float32, m x w): (ipiv[0:w] 1-based, info).
+    The tree of the library's tournament leaves: 256-row blocks, 8 children per node."""
+    blk = _fortran(np.asarray(a)[:, :w], np.float32)
+    m = blk.shape[0]
+    ipiv = np.zeros(max(w, 1), np.int32)
+    info = lib().orc_stslu_pivots(m, w, _ptr(blk), max(m, 1), node_rows, group, _ptr(ipiv))
+    return ipiv[: min(m, w)], info
 
 
 def dgetrf(a, pivot=True):

@@ -36,13 +36,14 @@ def _system(torch, n, cond, seed):
     return a, at, b, x_true
 
 
-@pytest.mark.parametrize("name", ["SB", "SH", "SS"])
+@pytest.mark.parametrize("name,tournament", [("SB", 0), ("SB", 1), ("SH", 0), ("SH", 1), ("SS", 0)])
 @pytest.mark.parametrize("n,cond", [(1024, 1e4), (2048, 1e6), (1100, 1e4), (4700, 1e5)])
-def test_gesv_refined(torch, name, n, cond):
+def test_gesv_refined(torch, name, tournament, n, cond):
     a, at, b, x_true = _system(torch, n, cond, seed=n)
     with A.Handle() as h:
         L = A.lib()
         L.aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 30, 30)
+        assert L.aclsolverSetTournamentPivoting(h.h, tournament) == A.SUCCESS
         lw = ctypes.c_size_t(0)
         assert getattr(L, f"aclsolver{name}gesv_bufferSize")(h.h, n, 1, ctypes.byref(lw)) == A.SUCCESS
         work = torch.empty(lw.value, dtype=torch.uint8, device="cuda")

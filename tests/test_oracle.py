@@ -286,3 +286,55 @@ def test_planted_mistakes_are_caught():
     col = np.array([[1.0], [-1.0], [0.5]], np.float32)
     _, p, _ = O.sgetrf(col)
     assert p[0] == 1
+
+
+# ---------------------------------------------------------------------------------------
+# Tournament pivot selection (no reference counterpart: the library's communication-
+# avoiding factorization). Pinned by (1) the single-node case, which is partial pivoting;
+# (2) a hand-worked two-level tree whose answer differs both from partial pivoting and
+# from a tree that stacked factored instead of original rows; (3) planted winners.
+# ---------------------------------------------------------------------------------------
+@pytest.mark.parametrize("m,w,seed", [(5, 3, 1), (100, 32, 2), (256, 32, 3), (256, 7, 4), (40, 40, 5)])
+def test_tslu_single_node_is_partial_pivoting(m, w, seed):
+    rng = np.random.default_rng(seed)
+    a = _rand(rng, m, w + 3)
+    piv, info = O.stslu_pivots(a, w)
+    _, ref, rinfo = O.sgetrf(a)
+    np.testing.assert_array_equal(piv, ref[:min(m, w)])
+    assert info == 0 and rinfo == 0
+
+
+def test_tslu_two_level_hand_example():
+    """node_rows 4, group 2, w 2 on rows (col0, col1):
+    r0 (7,-7) r1 (2,9) r2 (4,-1) r3 (-6,9) | r4 (0,-4) r5 (8,-6) r6 (-5,-6) r7 (9,3).
+    Node {r0..r3}: pivot r0, then col1 - (col0/7)(-7) = 11, 3, 3 -> keeps r0, r1.
+    Node {r4..r7}: pivot r7, then col1 - col0/3 = -4, -8.67, -4.33 -> keeps r7, r5.
+    Root on the ORIGINAL rows r0 r1 r7 r5: pivot r7, then col1 - col0/3 = -9.33, 8.33,
+    -8.67 -> r0. Interchanges: r7 from position 7 (ipiv 8), then r0, now at position 7
+    (ipiv 8). Partial pivoting over all rows would take r3 (11) second; stacking the
+    factored rows would take r1."""
+    a = np.array([[7, -7], [2, 9], [4, -1], [-6, 9], [0, -4], [8, -6], [-5, -6], [9, 3]], np.float32)
+    blk = np.asfortranarray(a)
+    ipiv = np.zeros(2, np.int32)
+    info = O.lib().orc_stslu_pivots(8, 2, blk.ctypes.data, 8, 4, 2, ipiv.ctypes.data)
+    assert info == 0
+    np.testing.assert_array_equal(ipiv, [8, 8])
+    _, ref, _ = O.sgetrf(a)
+    assert ref[1] != 8  # partial pivoting chooses differently
+
+
+def test_tslu_planted_winners():
+    """Row p_j holds 100 (j + 1) in column j and small entries elsewhere: every node keeps
+    the planted rows it holds, so the root's pivot order is p_0, p_1, ..."""
+    rng = np.random.default_rng(7)
+    m, w = 5000, 32
+    a = (rng.standard_normal((m, w)) * 0.01).astype(np.float32)
+    rows = rng.choice(m, w, replace=False)
+    for j, r in enumerate(rows):
+        a[r, j] = 100.0 * (j + 1)
+    piv, info = O.stslu_pivots(a, w)
+    assert info == 0
+    perm = np.arange(m)
+    for j, x in enumerate(piv - 1):
+        perm[[j, x]] = perm[[x, j]]
+    np.testing.assert_array_equal(perm[:w], rows)
