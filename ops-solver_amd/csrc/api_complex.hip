@@ -8,6 +8,7 @@
 #include <climits>
 #include <vector>
 
+#include "host_getrs.h"
 #include "internal.h"
 
 namespace ops
@@ -253,61 +254,56 @@ OPS_API aclsolverStatus_t aclsolverCgetrsDevice(aclsolverHandle_t handle, int tr
 }
 
 // Host-staged row-major cgetrs (src/cgetrs/cgetrs_host.cpp:151-231).
+namespace
+{
+// The HIP side of the staged host getrs (host_getrs.h) for complex64.
+struct ComplexStagingRuntime
+{
+    hipStream_t s;
+    int trans;
+    int n, nrhs;
+    int64_t lda, ldb;
+    int Malloc(void** p, size_t bytes)
+    {
+        if (hipMalloc(p, bytes) == hipSuccess) return 0;
+        (void)hipGetLastError();
+        return 1;
+    }
+    int Free(void* p) { return hipFree(p) != hipSuccess; }
+    int CopyToDevice(void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, s) != hipSuccess; }
+    int CopyToHost(void* h, const void* d, size_t b) { return hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, s) != hipSuccess; }
+    int ClearWord(int* p)
+    {
+        FillInfo(p, 1, 0, s);
+        return hipGetLastError() != hipSuccess;
+    }
+    int Synchronize() { return hipStreamSynchronize(s) != hipSuccess; }
+    size_t WorkBytes(int nn, int rr) { return ComplexGetrsWorkspaceBytes(nn, rr); }
+    int Solve(const hoststage::Staging& st)
+    {
+        float2* aRow = static_cast<float2*>(st.aRow);
+        float2* aCol = static_cast<float2*>(st.aCol);
+        float2* bRow = static_cast<float2*>(st.bRow);
+        float2* bCol = static_cast<float2*>(st.bCol);
+        hipError_t e = CTransposeRM(n, n, aRow, lda, aCol, n, s);
+        if (e == hipSuccess) e = CTransposeRM(n, nrhs, bRow, ldb, bCol, n, s);
+        if (e == hipSuccess) e = ComplexGetrs(trans, n, nrhs, aCol, n, st.piv, bCol, n, st.work, s);
+        if (e == hipSuccess) e = CTransposeRM(nrhs, n, bCol, n, bRow, ldb, s);
+        return e != hipSuccess;
+    }
+};
+}  // namespace
+
 OPS_API aclError aclsolverCgetrs(aclsolverHandle_t handle, const uint64_t trans, const int64_t n, const int64_t nrhs,
                                  aclFloatComplex* A, const int64_t lda, int32_t* ipiv, aclFloatComplex* B,
                                  const int64_t ldb, int32_t* info)
 {
     const int64_t lim = (handle != nullptr && H(handle)->strict) ? 8192 : 16384;
-    if (!(n > 0 && n <= lim && nrhs > 0 && nrhs <= lim && lda >= n && ldb >= nrhs && A != nullptr && B != nullptr &&
-          ipiv != nullptr && info != nullptr && lda - n <= 8191 && ldb - nrhs <= 8191 &&
-          (nrhs <= 4096 || ldb - nrhs <= 4095)))
-    {
-        fprintf(stderr, "[ERROR] aclsolverCgetrs invalid param: n, nrhs, lda, ldb or a null buffer.\n");
-        return ACL_ERROR_INVALID_PARAM;
-    }
-    if (trans > 2)
-    {
-        fprintf(stderr, "[ERROR] aclsolverCgetrs invalid param: trans must be 0, 1 or 2.\n");
-        return ACL_ERROR_INVALID_PARAM;
-    }
-    for (int64_t i = 0; i < n; ++i)
-        if (ipiv[i] < 1 || ipiv[i] > n)
-        {
-            *info = -4;
-            fprintf(stderr, "[ERROR] aclsolverCgetrs invalid ipiv[%lld] = %d, expected in [1, %lld].\n",
-                    static_cast<long long>(i), ipiv[i], static_cast<long long>(n));
-            return ACL_ERROR_INVALID_PARAM;
-        }
-    hipStream_t s = handle != nullptr ? Stream(handle) : nullptr;
-    const int N = static_cast<int>(n), R = static_cast<int>(nrhs);
-    const size_t aRow = static_cast<size_t>(n) * lda, bRow = static_cast<size_t>(n) * ldb;
-    const size_t bytes = (aRow + static_cast<size_t>(N) * N + bRow + static_cast<size_t>(N) * R) * sizeof(float2) +
-                         static_cast<size_t>(N) * sizeof(int) + ComplexGetrsWorkspaceBytes(N, R) + 1024;
-    char* base = nullptr;
-    if (hipMalloc(&base, bytes) != hipSuccess)
-    {
-        (void)hipGetLastError();
-        return ACL_ERROR_BAD_ALLOC;
-    }
-    float2* dArow = reinterpret_cast<float2*>(base);
-    float2* dA = dArow + aRow;
-    float2* dBrow = dA + static_cast<size_t>(N) * N;
-    float2* dB = dBrow + bRow;
-    int* dPiv = reinterpret_cast<int*>(dB + static_cast<size_t>(N) * R);
-    void* work = reinterpret_cast<char*>(dPiv + N) + 256 - (reinterpret_cast<uintptr_t>(dPiv + N) % 256);
-    hipError_t e = hipMemcpyAsync(dArow, A, aRow * sizeof(float2), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(dBrow, B, bRow * sizeof(float2), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(dPiv, ipiv, N * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = CTransposeRM(N, N, dArow, lda, dA, N, s);
-    if (e == hipSuccess) e = CTransposeRM(N, R, dBrow, ldb, dB, N, s);
-    if (e == hipSuccess) e = ComplexGetrs(static_cast<int>(trans), N, R, dA, N, dPiv, dB, N, work, s);
-    if (e == hipSuccess) e = CTransposeRM(R, N, dB, N, dBrow, ldb, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(B, dBrow, bRow * sizeof(float2), hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess) return ACL_ERROR_RT_FAILURE;
-    (void)hipFree(base);
-    if (e != hipSuccess) return ACL_ERROR_RT_FAILURE;
-    *info = 0;
-    return ACL_SUCCESS;
+    const hoststage::Limits limits{lim, 8191, 8191, 4096, 4095, 2};
+    ComplexStagingRuntime rt{handle != nullptr ? Stream(handle) : nullptr, static_cast<int>(trans),
+                             static_cast<int>(n), static_cast<int>(nrhs), lda, ldb};
+    return hoststage::StagedGetrs(rt, "aclsolverCgetrs", limits, sizeof(float2), trans, n, nrhs, A, lda, ipiv, B, ldb,
+                                  info);
 }
 
 static size_t CGetriWorkBytes(int n)

@@ -8,6 +8,7 @@
 #include <climits>
 #include <vector>
 
+#include "host_getrs.h"
 #include "internal.h"
 
 namespace ops
@@ -271,72 +272,60 @@ OPS_API aclsolverStatus_t aclsolverSgetrsDevice(aclsolverHandle_t handle, int tr
 // ---------------------------------------------------------------------------------------
 // Host-staged row-major sgetrs (src/sgetrs/sgetrs_host.cpp:131-195)
 // ---------------------------------------------------------------------------------------
+namespace
+{
+// The HIP side of the staged host getrs (host_getrs.h): copies and launches on the handle's
+// stream, the real solve between the two transposes.
+struct RealStagingRuntime
+{
+    hipStream_t s;
+    int trans;
+    int n, nrhs;
+    int64_t lda, ldb;
+    int Malloc(void** p, size_t bytes)
+    {
+        if (hipMalloc(p, bytes) == hipSuccess) return 0;
+        (void)hipGetLastError();
+        return 1;
+    }
+    int Free(void* p) { return hipFree(p) != hipSuccess; }
+    int CopyToDevice(void* d, const void* h, size_t b) { return hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, s) != hipSuccess; }
+    int CopyToHost(void* h, const void* d, size_t b) { return hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, s) != hipSuccess; }
+    int ClearWord(int* p)
+    {
+        FillInfo(p, 1, 0, s);
+        return hipGetLastError() != hipSuccess;
+    }
+    int Synchronize() { return hipStreamSynchronize(s) != hipSuccess; }
+    size_t WorkBytes(int nn, int rr) { return RealGetrsWorkspaceBytes(nn, rr); }
+    int Solve(const hoststage::Staging& st)
+    {
+        float* aRow = static_cast<float*>(st.aRow);
+        float* aCol = static_cast<float*>(st.aCol);
+        float* bRow = static_cast<float*>(st.bRow);
+        float* bCol = static_cast<float*>(st.bCol);
+        // Row-major (r, c) at X[r*ld + c] is column-major X^T: transpose into column-major.
+        hipError_t e = TransposeRM(n, n, aRow, lda, aCol, n, s);
+        if (e == hipSuccess) e = TransposeRM(n, nrhs, bRow, ldb, bCol, n, s);
+        if (e == hipSuccess) e = RealGetrs(trans, n, nrhs, aCol, n, st.piv, bCol, n, st.work, s, st.wait);
+        // back: column-major (n x nrhs, ld n) -> row-major with ldb (the transpose of an
+        // nrhs x n row-major view).
+        if (e == hipSuccess) e = TransposeRM(nrhs, n, bCol, n, bRow, ldb, s);
+        return e != hipSuccess;
+    }
+};
+}  // namespace
+
 OPS_API aclError aclsolverSgetrs(aclsolverHandle_t handle, const uint64_t trans, const int64_t n, const int64_t nrhs,
                                  float* A, const int64_t lda, int32_t* ipiv, float* B, const int64_t ldb,
                                  int32_t* info)
 {
     const int64_t lim = (handle != nullptr && H(handle)->strict) ? 8192 : kEngineMaxOrder;
-    if (!(n > 0 && n <= lim && nrhs > 0 && nrhs <= lim && lda >= n && ldb >= nrhs && A != nullptr && B != nullptr &&
-          ipiv != nullptr && info != nullptr && lda - n <= 16383 && ldb - nrhs <= 16383))
-    {
-        fprintf(stderr, "[ERROR] aclsolverSgetrs invalid param: n, nrhs, lda, ldb or a null buffer.\n");
-        return ACL_ERROR_INVALID_PARAM;
-    }
-    if (trans > 1)
-    {
-        fprintf(stderr, "[ERROR] aclsolverSgetrs invalid param: trans must be 0 or 1.\n");
-        return ACL_ERROR_INVALID_PARAM;
-    }
-    for (int64_t i = 0; i < n; ++i)  // src/utils/pivot_check.h:27-42
-        if (ipiv[i] < 1 || ipiv[i] > n)
-        {
-            *info = -4;
-            fprintf(stderr, "[ERROR] aclsolverSgetrs invalid ipiv[%lld] = %d, expected in [1, %lld].\n",
-                    static_cast<long long>(i), ipiv[i], static_cast<long long>(n));
-            return ACL_ERROR_INVALID_PARAM;
-        }
-    hipStream_t s = handle != nullptr ? Stream(handle) : nullptr;
-    const int N = static_cast<int>(n), R = static_cast<int>(nrhs);
-    const size_t aRow = static_cast<size_t>(n) * lda, bRow = static_cast<size_t>(n) * ldb;
-    const size_t bytes = (aRow + static_cast<size_t>(N) * N + bRow + static_cast<size_t>(N) * R) * sizeof(float) +
-                         static_cast<size_t>(N + 1) * sizeof(int) + RealGetrsWorkspaceBytes(N, R) + 1024;
-    char* base = nullptr;
-    if (hipMalloc(&base, bytes) != hipSuccess)
-    {
-        (void)hipGetLastError();
-        return ACL_ERROR_BAD_ALLOC;
-    }
-    float* dArow = reinterpret_cast<float*>(base);
-    float* dA = dArow + aRow;
-    float* dBrow = dA + static_cast<size_t>(N) * N;
-    float* dB = dBrow + bRow;
-    int* dPiv = reinterpret_cast<int*>(dB + static_cast<size_t>(N) * R);
-    int* dInfo = dPiv + N;  // the solve's wait-timeout report
-    void* work = reinterpret_cast<char*>(dInfo + 1) + 256 - (reinterpret_cast<uintptr_t>(dInfo + 1) % 256);
-    FillInfo(dInfo, 1, 0, s);
-    hipError_t e = hipMemcpyAsync(dArow, A, aRow * sizeof(float), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(dBrow, B, bRow * sizeof(float), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(dPiv, ipiv, N * sizeof(int), hipMemcpyHostToDevice, s);
-    // Row-major (r, c) at X[r*ld + c] is column-major X^T: transpose into column-major.
-    if (e == hipSuccess) e = TransposeRM(N, N, dArow, lda, dA, N, s);
-    if (e == hipSuccess) e = TransposeRM(N, R, dBrow, ldb, dB, N, s);
-    if (e == hipSuccess) e = RealGetrs(static_cast<int>(trans), N, R, dA, N, dPiv, dB, N, work, s, dInfo);
-    // back: column-major (N x R, ld N) -> row-major with ldb: transpose of an R x N row-major view.
-    if (e == hipSuccess) e = TransposeRM(R, N, dB, N, dBrow, ldb, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(B, dBrow, bRow * sizeof(float), hipMemcpyDeviceToHost, s);
-    int waitInfo = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&waitInfo, dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipError_t sync = hipStreamSynchronize(s);
-    if (sync != hipSuccess) return ACL_ERROR_RT_FAILURE;  // buffers retained: work may still be in flight
-    (void)hipFree(base);
-    if (e != hipSuccess) return ACL_ERROR_RT_FAILURE;
-    if (waitInfo != 0)
-    {
-        fprintf(stderr, "[ERROR] aclsolverSgetrs: a bounded wait of the triangular solve gave up.\n");
-        return ACL_ERROR_RT_FAILURE;
-    }
-    *info = 0;
-    return ACL_SUCCESS;
+    const hoststage::Limits limits{lim, 16383, 16383, lim, 16383, 1};
+    RealStagingRuntime rt{handle != nullptr ? Stream(handle) : nullptr, static_cast<int>(trans),
+                          static_cast<int>(n), static_cast<int>(nrhs), lda, ldb};
+    return hoststage::StagedGetrs(rt, "aclsolverSgetrs", limits, sizeof(float), trans, n, nrhs, A, lda, ipiv, B, ldb,
+                                  info);
 }
 
 // ---------------------------------------------------------------------------------------
