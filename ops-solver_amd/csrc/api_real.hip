@@ -10,6 +10,7 @@
 
 #include "host_getrs.h"
 #include "internal.h"
+#include "lu_kernels.h"
 
 namespace ops
 {
@@ -442,6 +443,22 @@ OPS_API aclsolverStatus_t aclsolverSgetriBatched(aclsolverHandle_t handle, int n
     {
         if (GetriSmallBatched(n, Aarray, lda, PivotArray, Carray, ldc, infoArray, batchSize, s) != hipSuccess)
             return ACLSOLVER_STATUS_EXECUTION_FAILED;
+        return Launched();
+    }
+    if (n <= 512)
+    {
+        // Blocked batched inverse, in chunks whose diagonal-block inverses fit 64 MiB.
+        const size_t per = GetriBlockedBatchedWorkspaceBytes(n, 1);
+        const int chunk = static_cast<int>(std::max<size_t>(1, std::min<size_t>(65535, (64u << 20) / per)));
+        float* work = static_cast<float*>(HandleScratch(handle, per * std::min(chunk, batchSize)));
+        if (work == nullptr) return ACLSOLVER_STATUS_ALLOC_FAILED;
+        for (int b0 = 0; b0 < batchSize; b0 += chunk)
+        {
+            const int nb = std::min(chunk, batchSize - b0);
+            if (GetriBlockedBatched(n, Aarray + b0, lda, PivotArray ? PivotArray + static_cast<size_t>(b0) * n : nullptr,
+                                    Carray + b0, ldc, infoArray + b0, nb, work, s) != hipSuccess)
+                return ACLSOLVER_STATUS_EXECUTION_FAILED;
+        }
         return Launched();
     }
     // Larger orders: per matrix, solve the factors against the identity straight into C.

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 
 #include "lu_kernels.h"
 
@@ -690,6 +691,200 @@ __global__ __launch_bounds__(256) void trsm_lu_kernel(int K, int N, const float*
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Batched inverse from LU factors (getri_batched, orders 65..512): C = U^{-1} L^{-1} P,
+// reference src/sgetri/real_lu_blocked_inverse_kernel.cpp:1115-1301 computes the same
+// inverse blockwise per matrix; here every matrix of the batch moves through three launches
+// together: the row permutation P into C, the inverses of the 64 x 64 diagonal blocks of L
+// (unit lower) and U (upper), then the two block substitutions on FP32 MFMA.
+// ---------------------------------------------------------------------------------------
+
+// C(:, :) = P, P(i, perm(i)) = 1 where perm is ipiv's interchange sequence applied to the
+// identity (row i of P A is row perm(i) of A); no pivots: the identity. One workgroup per
+// matrix: the interchanges run on an LDS index array, then the columns are written.
+__global__ __launch_bounds__(256) void getri_perm_identity_kernel(int n, const int* __restrict__ piv,
+                                                                  float* const* __restrict__ Carr, int ldc)
+{
+    extern __shared__ int s_perm[];
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < n; i += 256) s_perm[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0 && piv != nullptr)
+    {
+        const int* p = piv + static_cast<size_t>(b) * n;
+        for (int k = 0; k < n; ++k)
+        {
+            const int q = p[k] - 1;
+            if (q != k && q >= 0 && q < n)
+            {
+                const int t = s_perm[k];
+                s_perm[k] = s_perm[q];
+                s_perm[q] = t;
+            }
+        }
+    }
+    __syncthreads();
+    float* C = Carr[b];
+    for (int c = 0; c < n; ++c)
+        for (int i = threadIdx.x; i < n; i += 256) C[static_cast<size_t>(c) * ldc + i] = s_perm[i] == c ? 1.0f : 0.0f;
+}
+
+// info[b] = first zero U(k, k) + 1 (0: none), as getri reports a singular factor.
+__global__ __launch_bounds__(256) void getri_first_zero_kernel(int n, const float* const* __restrict__ Aarr, int lda,
+                                                               int* __restrict__ info)
+{
+    __shared__ int s_first;
+    if (threadIdx.x == 0) s_first = INT_MAX;
+    __syncthreads();
+    const float* A = Aarr[blockIdx.x];
+    for (int k = threadIdx.x; k < n; k += 256)
+        if (A[static_cast<size_t>(k) * lda + k] == 0.0f) atomicMin(&s_first, k);
+    __syncthreads();
+    if (threadIdx.x == 0) info[blockIdx.x] = s_first == INT_MAX ? 0 : s_first + 1;
+}
+
+// Inverses of the 64 x 64 diagonal blocks of every matrix's unit-lower L (UPPER = false)
+// or upper U (UPPER = true): workgroup (block bi, matrix b) writes the inverse to
+// inv + (b * nblk + bi) * 4096, column-major, ld 64, identity beyond the block. Column c is
+// the substitution of e_c, a quad of lanes per column (16 rows each), as trtri_unit_lower64.
+template <bool UPPER>
+__global__ __launch_bounds__(256) void trtri_diag64_batched_kernel(int K, const float* const* __restrict__ Tarr,
+                                                                   int ldt, float* __restrict__ inv)
+{
+    __shared__ __attribute__((aligned(16))) float Ts[64][64];  // Ts[s][i] = T(i, s) of the block
+    const int nblk = (K + 63) / 64;
+    const int bi = blockIdx.x, k0 = bi * 64, kb = min(64, K - k0);
+    const float* Tb = Tarr[blockIdx.y] + static_cast<size_t>(k0) * ldt + k0;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
+    {
+        const int s2 = e >> 6, i = e & 63;
+        Ts[s2][i] = (i < kb && s2 < kb) ? Tb[static_cast<size_t>(s2) * ldt + i] : 0.0f;
+    }
+    __syncthreads();
+    const int col = threadIdx.x >> 2, q = threadIdx.x & 3;
+    float x[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x[t] = (16 * q + t == col) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 64; ++u)
+    {
+        const int s2 = UPPER ? 63 - u : u;
+        if (s2 >= kb) continue;  // uniform
+        const int o = s2 >> 4, so = s2 & 15;
+        float xs;
+        if (o == 0)
+            xs = QuadBroadcast<0x00>(x[so]);
+        else if (o == 1)
+            xs = QuadBroadcast<0x55>(x[so]);
+        else if (o == 2)
+            xs = QuadBroadcast<0xAA>(x[so]);
+        else
+            xs = QuadBroadcast<0xFF>(x[so]);
+        if (UPPER)
+        {
+            xs = xs / Ts[s2][s2];
+            if (q == o) x[so] = xs;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+        {
+            const int row = 16 * q + t;
+            const bool upd = UPPER ? row < s2 : (row > s2 && row < kb);
+            const float nx = fmaf(-Ts[s2][row], xs, x[t]);
+            x[t] = upd ? nx : x[t];
+        }
+    }
+    float* out = inv + (static_cast<size_t>(blockIdx.y) * nblk + bi) * 4096 + col * 64 + 16 * q;
+#pragma unroll
+    for (int t = 0; t < 16; t += 4)
+        *reinterpret_cast<float4*>(out + t) = make_float4(x[t], x[t + 1], x[t + 2], x[t + 3]);
+}
+
+// B <- T^{-1} B for every matrix of the batch (T unit lower from L, or upper from U; K <=
+// 512): workgroup (strip, matrix) holds a 32-column strip of B in LDS; per 64-row block,
+// in ascending (lower) or descending (upper) order: X_b = inv(T_bb) B_b, then the other
+// unsolved rows -= T(rows, b) X_b, both on FP32 MFMA -- trsm_lu_kernel's scheme.
+template <bool UPPER>
+__global__ __launch_bounds__(256) void trsm_blocked_batched_kernel(int K, int N, const float* const* __restrict__ Tarr,
+                                                                   int ldt, const float* __restrict__ inv,
+                                                                   float* const* __restrict__ Barr, int ldb)
+{
+    __shared__ float Xs[kTrsmLuMaxK][33];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float* T = Tarr[blockIdx.y];
+    float* B = Barr[blockIdx.y];
+    const int c0 = blockIdx.x * 32, nc = min(32, N - c0);
+    const int Kp = (K + 63) & ~63;
+    const int nblk = Kp / 64;
+    const float* Inv = inv + static_cast<size_t>(blockIdx.y) * nblk * 4096;
+    for (int e = tid; e < 32 * Kp; e += 256)
+    {
+        const int c = e / Kp, r = e - c * Kp;
+        Xs[r][c] = (c < nc && r < K) ? B[static_cast<size_t>(c0 + c) * ldb + r] : 0.0f;
+    }
+    __syncthreads();
+    const int li = lane & 31, lk = lane >> 5;
+    float lf[32];
+    auto loadTile = [&](int k0, int k1, int r0)
+    {
+        const int row = r0 + li;
+        const float* Lc = T + static_cast<size_t>(k0) * ldt + row;
+#pragma unroll
+        for (int t = 0; t < 32; ++t)
+        {
+            const int k = 2 * t + lk;
+            lf[t] = (row < K && k0 + k < k1) ? Lc[static_cast<size_t>(k) * ldt] : 0.0f;
+        }
+    };
+    for (int q = 0; q < nblk; ++q)
+    {
+        const int b = UPPER ? nblk - 1 - q : q;
+        const int k0 = b * 64, k1 = min(K, k0 + 64);
+        // rows still to update: below the block (lower) or above it (upper)
+        const int rLo = UPPER ? 0 : k1, rHi = UPPER ? k0 : K;
+        float lb[32];
+        if (wave < 2)
+        {
+            const float* Lb = Inv + static_cast<size_t>(b) * 4096 + wave * 32 + li;
+#pragma unroll
+            for (int t = 0; t < 32; ++t) lb[t] = Lb[(2 * t + lk) * 64];
+        }
+        int r0 = rLo + wave * 32;
+        if (r0 < rHi) loadTile(k0, k1, r0);
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+        if (wave < 2)
+#pragma unroll
+            for (int t = 0; t < 32; ++t)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Xs[k0 + 2 * t + lk][li], lb[t], acc, 0, 0, 0);
+        __syncthreads();
+        if (wave < 2)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) Xs[k0 + wave * 32 + li][(v & 3) + 8 * (v >> 2) + 4 * lk] = acc[v];
+        __syncthreads();
+        for (; r0 < rHi; r0 += 128)
+        {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 32; ++t)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Xs[k0 + 2 * t + lk][li], lf[t], acc, 0, 0, 0);
+            const int row = r0 + li;
+            if (r0 + 128 < rHi) loadTile(k0, k1, r0 + 128);
+            if (row < rHi)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) Xs[row][(v & 3) + 8 * (v >> 2) + 4 * lk] -= acc[v];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 32 * Kp; e += 256)
+    {
+        const int c = e / Kp, r = e - c * Kp;
+        if (c < nc && r < K) B[static_cast<size_t>(c0 + c) * ldb + r] = Xs[r][c];
+    }
+}
+
 hipError_t TrsmDiag(int kb, int N, const float* T, int ldt, bool fwd, bool unit, bool transT, float* B, int ldb,
                     hipStream_t stream)
 {
@@ -775,6 +970,30 @@ hipError_t TrsmLowerUnitApply(int K, int N, const float* T, int ldt, const float
     if (K <= 0 || N <= 0) return hipSuccess;
     if (K > kTrsmLuMaxK) return hipErrorInvalidValue;
     hipLaunchKernelGGL(trsm_lu_kernel, dim3((N + 31) / 32), dim3(256), 0, stream, K, N, T, ldt, linv, B, ldb);
+    return hipGetLastError();
+}
+
+size_t GetriBlockedBatchedWorkspaceBytes(int n, int batch)
+{
+    return static_cast<size_t>(2) * ((n + 63) / 64) * 4096 * sizeof(float) * static_cast<size_t>(batch > 0 ? batch : 0);
+}
+
+hipError_t GetriBlockedBatched(int n, const float* const* Aarr, int lda, const int* piv, float* const* Carr, int ldc,
+                               int* info, int batch, float* work, hipStream_t stream)
+{
+    if (n <= 0 || batch <= 0) return hipSuccess;
+    if (n > kTrsmLuMaxK || batch > 65535) return hipErrorInvalidValue;
+    const int nblk = (n + 63) / 64;
+    float* linv = work;
+    float* uinv = work + static_cast<size_t>(batch) * nblk * 4096;
+    hipLaunchKernelGGL(getri_first_zero_kernel, dim3(batch), dim3(256), 0, stream, n, Aarr, lda, info);
+    hipLaunchKernelGGL(getri_perm_identity_kernel, dim3(batch), dim3(256), static_cast<size_t>(n) * sizeof(int), stream,
+                       n, piv, Carr, ldc);
+    hipLaunchKernelGGL(trtri_diag64_batched_kernel<false>, dim3(nblk, batch), dim3(256), 0, stream, n, Aarr, lda, linv);
+    hipLaunchKernelGGL(trtri_diag64_batched_kernel<true>, dim3(nblk, batch), dim3(256), 0, stream, n, Aarr, lda, uinv);
+    const dim3 grid((n + 31) / 32, batch);
+    hipLaunchKernelGGL(trsm_blocked_batched_kernel<false>, grid, dim3(256), 0, stream, n, n, Aarr, lda, linv, Carr, ldc);
+    hipLaunchKernelGGL(trsm_blocked_batched_kernel<true>, grid, dim3(256), 0, stream, n, n, Aarr, lda, uinv, Carr, ldc);
     return hipGetLastError();
 }
 

@@ -37,4 +37,12 @@ hipError_t TrsmLowerUnitInverse(int K, const float* T, int ldt, float* linv, hip
 hipError_t TrsmLowerUnitApply(int K, int N, const float* T, int ldt, const float* linv, float* B, int ldb,
                               hipStream_t stream);
 
+// Batched inverse from LU factors for orders up to 512: C_b = U_b^{-1} L_b^{-1} P_b for
+// every matrix (pointer arrays), info_b = first zero U(k, k) + 1. piv (batch * n, 1-based)
+// may be null (no pivoting). work: GetriBlockedBatchedWorkspaceBytes(n, batch); at most
+// 65535 matrices per call.
+size_t GetriBlockedBatchedWorkspaceBytes(int n, int batch);
+hipError_t GetriBlockedBatched(int n, const float* const* Aarr, int lda, const int* piv, float* const* Carr, int ldc,
+                               int* info, int batch, float* work, hipStream_t stream);
+
 }  // namespace ops

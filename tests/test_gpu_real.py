@@ -313,7 +313,8 @@ def test_smatinv_strided_ipiv_and_singular(torch, handle):
             assert O.inverse_residual(mats[b], inv) <= 5
 
 
-@pytest.mark.parametrize("n,batch", [(32, 64), (5, 9), (64, 8), (100, 2)])
+@pytest.mark.parametrize("n,batch", [(32, 64), (5, 9), (64, 8), (100, 2), (65, 33), (128, 20), (200, 7), (512, 3),
+                                     (600, 2)])
 def test_sgetri_batched(torch, handle, n, batch):
     rng = np.random.default_rng(n + 77)
     mats = [(rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32) for _ in range(batch)]
@@ -329,6 +330,29 @@ def test_sgetri_batched(torch, handle, n, batch):
     for m, c in zip(mats, dC):
         inv = A.from_device_cm(c, n, n)
         assert O.inverse_residual(m, inv) <= 5
+
+
+def test_sgetri_batched_blocked_singular_and_unpivoted(torch, handle):
+    """Orders 65..512 (blocked batched inverse): a singular factor reports info = first zero
+    U(k, k) + 1 while the others are inverted; factors without pivots (PivotArray null)."""
+    n, batch = 130, 5
+    rng = np.random.default_rng(1300)
+    mats = [(rng.standard_normal((n, n)) / np.sqrt(n) + 3 * np.eye(n)).astype(np.float32) for _ in range(batch)]
+    facts = [O.sgetrf(m, pivot=False) for m in mats]
+    lus = [f[0].copy() for f in facts]
+    lus[2][97, 97] = 0.0  # matrix 2: U(97, 97) = 0
+    dA = [A.to_device_cm(lu) for lu in lus]
+    dC = [torch.zeros((n, n), device="cuda") for _ in range(batch)]
+    info = torch.full((batch,), -7, dtype=torch.int32, device="cuda")
+    pa, pc = _ptr_array(torch, dA), _ptr_array(torch, dC)
+    st = A.lib().aclsolverSgetriBatched(handle.h, n, A.ptr(pa), n, None, A.ptr(pc), n, A.ptr(info), batch)
+    torch.cuda.synchronize()
+    assert st == A.SUCCESS
+    got = info.cpu().numpy()
+    assert got[2] == 98 and (np.delete(got, 2) == 0).all()
+    for b in (0, 1, 3, 4):
+        inv = A.from_device_cm(dC[b], n, n)
+        assert O.inverse_residual(mats[b], inv) <= 5
 
 
 def test_sgetri_single(torch, handle):
