@@ -118,30 +118,42 @@ def time_mode(A, torch, h, name, steps, warmup, n, at, b, x, work, lwork, dipiv,
     t1 = time.perf_counter()
     prof = None
     if profile:
-        launches, work_f, ms = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
-        A.lib().aclsolverProfileEnd(0, ctypes.byref(launches), ctypes.byref(work_f), ctypes.byref(ms))
-        prof = (launches.value, work_f.value, ms.value)
+        prof = {}
+        for kind, key in ((0, "gemm"), (3, "leaf")):
+            launches, work_f, ms = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+            A.lib().aclsolverProfileEnd(kind, ctypes.byref(launches), ctypes.byref(work_f), ctypes.byref(ms))
+            prof[key] = (launches.value, work_f.value, ms.value)
     info = int(dinfo.item())
     return (t1 - t0) * 1e3 / steps, iters, info, prof
 
 
-def cpu_baseline(n_cpu):
-    """Host LAPACK sgetrf+sgetrs (SciPy/OpenBLAS, the CPU path of the reference's tests)
-    on a bounded sample: one n_cpu system, median of 3 after 1 warm-up."""
-    import numpy as np
-    import scipy.linalg.lapack as la
-
+def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
 
-        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        return max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
     except Exception:
-        threads = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
+
+
+def _host_workers():
+    # the host share of one GPU on the box (16 cores), or fewer where fewer are available
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(n_cpu, reps=3):
+    """Host LAPACK sgetrf+sgetrs (SciPy/OpenBLAS, the CPU path of the reference's tests)
+    on one N(0,1)/sqrt(n) system of order n_cpu: median of reps runs after 1 warm-up, on
+    every BLAS thread."""
+    import numpy as np
+    import scipy.linalg.lapack as la
+
+    threads = _blas_threads()
     rng = np.random.default_rng(4)
     a = np.asfortranarray(rng.standard_normal((n_cpu, n_cpu)).astype(np.float32) / np.sqrt(n_cpu))
     b = rng.standard_normal((n_cpu, 1)).astype(np.float32)
     times = []
-    for rep in range(4):
+    for rep in range(reps + 1):
         t0 = time.perf_counter()
         lu, piv, info = la.sgetrf(a)
         x, info2 = la.sgetrs(lu, piv, b)
@@ -151,8 +163,144 @@ def cpu_baseline(n_cpu):
     t = sorted(times)[len(times) // 2]
     flops = 2.0 / 3.0 * n_cpu ** 3 + 2.0 * n_cpu ** 2
     return {"value": round(flops / t / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads), "kind": "reference",
+            "ms": round(t * 1e3, 3),
             "sample": f"host LAPACK (SciPy OpenBLAS) sgetrf+sgetrs, one N(0,1)/sqrt(n) system of n={n_cpu}, "
-                      f"median of 3 runs ({t:.2f} s each) on {threads} threads"}
+                      f"median of {reps} runs ({t:.3f} s each) on {threads} threads"}
+
+
+def cpu_complex_baseline(n=8192, nrhs=16, reps=2):
+    """Host LAPACK cgetrf+cgetrs (SciPy/OpenBLAS) on the complex workload's distribution."""
+    import numpy as np
+    import scipy.linalg.lapack as la
+
+    threads = _blas_threads()
+    rng = np.random.default_rng(5)
+    a = ((rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * np.sqrt(0.5 / n)).astype(np.complex64)
+    a[np.diag_indices(n)] += 1.5
+    a = np.asfortranarray(a)
+    b = ((rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))).astype(np.complex64))
+    times = []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        lu, piv, info = la.cgetrf(a)
+        x, info2 = la.cgetrs(lu, piv, b)
+        t = time.perf_counter() - t0
+        if rep:
+            times.append(t)
+    t = sorted(times)[len(times) // 2]
+    flops = 8.0 / 3.0 * n ** 3 + 8.0 * n * n * nrhs
+    return {"value": round(flops / t / 1e9, 2), "unit": "GFLOP/s", "cores": int(threads), "kind": "reference",
+            "ms": round(t * 1e3, 3),
+            "sample": f"host LAPACK (SciPy OpenBLAS) cgetrf+cgetrs, n={n}, {nrhs} RHS, median of {reps} runs "
+                      f"({t:.2f} s each) on {threads} threads"}
+
+
+def _parallel_over_matrices(fn, items):
+    """Runs fn(chunk) over chunks of items on a thread pool with single-threaded BLAS (the
+    LAPACK calls release the GIL), as OpenMP over matrices would."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import contextlib
+
+    workers = _host_workers()
+    chunks = [items[i::workers] for i in range(workers)]
+    try:
+        from threadpoolctl import threadpool_limits
+
+        ctx = threadpool_limits(limits=1, user_api="blas")
+    except ImportError:
+        ctx = contextlib.nullcontext()
+    with ctx:
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(fn, chunks))
+        return time.perf_counter() - t0, workers
+
+
+def cpu_batched_inverse_baseline(sample=16384, n=32):
+    """Host LAPACK inverses (numpy linalg.inv: sgesv per matrix) of `sample` matrices of the
+    uniform-batch distribution, split over the host cores with single-threaded LAPACK."""
+    import numpy as np
+
+    rng = np.random.default_rng(32)
+    a = (rng.standard_normal((sample, n, n)) / np.sqrt(n) + 2 * np.eye(n)).astype(np.float32)
+    idx = list(range(0, sample, 256))
+
+    def run(starts):
+        for s0 in starts:
+            np.linalg.inv(a[s0:s0 + 256])
+
+    run(idx[:1])  # warm-up
+    t, workers = _parallel_over_matrices(run, idx)
+    return {"value": round(sample / t, 1), "unit": "matrices/s", "cores": workers, "kind": "reference",
+            "sample": f"numpy linalg.inv (LAPACK sgesv) on {sample} of the 65536 32x32 matrices, "
+                      f"{workers} threads over matrices, single-threaded LAPACK each ({t:.2f} s)"}
+
+
+def cpu_uneven_baseline(sizes, nrhs=4, sample=2000):
+    """Host LAPACK sgetrf+sgetrs (SciPy) on the first `sample` systems of the uneven batch,
+    split over the host cores with single-threaded LAPACK."""
+    import numpy as np
+    import scipy.linalg.lapack as la
+
+    rng = np.random.default_rng(20001)
+    mats = []
+    for ni in sizes[:sample]:
+        ni = int(ni)
+        a = np.asfortranarray((rng.standard_normal((ni, ni)) / np.sqrt(ni) + 2 * np.eye(ni)).astype(np.float32))
+        b = np.asfortranarray(rng.standard_normal((ni, nrhs)).astype(np.float32))
+        mats.append((a, b))
+
+    def run(chunk):
+        for a, b in chunk:
+            lu, piv, info = la.sgetrf(a)
+            la.sgetrs(lu, piv, b)
+
+    run(mats[:4])
+    t, workers = _parallel_over_matrices(run, mats)
+    return {"value": round(sample / t, 1), "unit": "solves/s", "cores": workers, "kind": "reference",
+            "sample": f"SciPy LAPACK sgetrf+sgetrs on the first {sample} of the 20000 systems (n_i in [8, 256], "
+                      f"4 RHS), {workers} threads over systems, single-threaded LAPACK each ({t:.2f} s)"}
+
+
+def small_dense(A, torch, h, n=512, steps=20):
+    """The plain FP32 sgetrf + sgetrs at n = 512 (N(0,1)/sqrt(n), one RHS), drop-in entry
+    points: sgetrf (with its finite scan) and the Device getrs; A restored outside the
+    events each step."""
+    import numpy as np
+
+    L = A.lib()
+    rng = np.random.default_rng(512)
+    a0 = torch.from_numpy((rng.standard_normal((n, n)) / np.sqrt(n)).astype(np.float32)).cuda()
+    b0 = torch.from_numpy(rng.standard_normal(n).astype(np.float32)).cuda()
+    a, b = a0.clone(), b0.clone()
+    lw = ctypes.c_int(0)
+    assert L.aclsolverSgetrf_bufferSize(h.h, n, n, A.ptr(a), n, ctypes.byref(lw)) == A.SUCCESS
+    work = torch.empty(lw.value, device="cuda")
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    info = torch.empty(1, dtype=torch.int32, device="cuda")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for s in range(steps + 2):
+        a.copy_(a0)
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        ev0.record()
+        assert L.aclsolverSgetrf(h.h, n, n, A.ptr(a), n, A.ptr(work), A.ptr(ipiv), A.ptr(info)) == A.SUCCESS
+        assert L.aclsolverSgetrsDevice(h.h, 0, n, 1, A.ptr(a), n, A.ptr(ipiv), A.ptr(b), n, A.ptr(info)) == A.SUCCESS
+        ev1.record()
+        torch.cuda.synchronize()
+        if s >= 2:
+            times.append(ev0.elapsed_time(ev1))
+    ms = sorted(times)[len(times) // 2]
+    a64 = a0.T.double()
+    x64 = b.double()
+    r = b0.double() - a64 @ x64
+    berr = (r.abs().max() / (a64.abs().sum(1).max() * x64.abs().max() + b0.double().abs().max())).item()
+    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    return {"workload": f"sgetrf+sgetrs FP32 n={n}, N(0,1)/sqrt(n), 1 RHS (drop-in sgetrf incl. finite scan)",
+            "ms": round(ms, 4), "GFLOP_per_s": round(flops / ms / 1e6, 1), "backward_error": berr,
+            "info": int(info.item())}
 
 
 def batched_inverse(A, torch, h, steps=20):
@@ -181,6 +329,22 @@ def batched_inverse(A, torch, h, steps=20):
     ev1.record()
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / steps
+    # The drop-in entry point: Device arrays of Device pointers (pointer-array validation
+    # included in the timed calls).
+    pa = (a.data_ptr() + torch.arange(batch, device="cuda", dtype=torch.int64) * n * n * 4)
+    pc = (c.data_ptr() + torch.arange(batch, device="cuda", dtype=torch.int64) * n * n * 4)
+
+    def run_ptr():
+        st = L.aclsolverSmatinvBatched(h.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
+        assert st == A.SUCCESS
+
+    run_ptr()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_ptr()
+    torch.cuda.synchronize()
+    ms_ptr = (time.perf_counter() - t0) * 1e3 / steps
     # check a sample against A C = I
     idx = torch.arange(0, batch, 4093, device="cuda")
     res = (torch.bmm(a[idx].transpose(1, 2).contiguous().double(), c[idx].transpose(1, 2).contiguous().double())
@@ -189,7 +353,9 @@ def batched_inverse(A, torch, h, steps=20):
     return {"workload": "smatinv 65536 x 32x32 FP32, A_i = G_i/sqrt(32) + 2I (output inverse + ipiv + info)",
             "ms": round(ms, 4), "matrices_per_s": round(batch / ms * 1e3, 1),
             "GB_per_s": round(bytes_ / ms / 1e6, 1), "hbm_frac": round(bytes_ / ms / 1e-3 / PEAK_HBM, 4),
-            "max_abs_AC_minus_I_sample": res, "all_info_zero": bool((info == 0).all().item())}
+            "max_abs_AC_minus_I_sample": res, "all_info_zero": bool((info == 0).all().item()),
+            "dropin_pointer_array_ms": round(ms_ptr, 4),
+            "dropin_note": "aclsolverSmatinvBatched (pointer arrays, host wall clock per call incl. validation)"}
 
 
 def complex_solve(A, torch, h, n=8192, nrhs=16, steps=2):
@@ -339,10 +505,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--cpu-n", type=int, default=8192)
+    ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
-    ap.add_argument("--modes", default="SS,SB", help="gesv variants to time: SS, SH, SB (SB, the headline, always runs)")
+    ap.add_argument("--modes", default="SS,SH,SB",
+                    help="gesv variants to time: SS, SH, SB (SB, the headline, always runs)")
     ap.add_argument("--restart", type=int, default=60)
     ap.add_argument("--matrix", choices=["qr", "shifted"], default="qr",
                     help="qr: Q1 diag(s) Q2^T with cond 1e6 (the workload); shifted: N(0,1)/sqrt(n) + 2I")
@@ -390,13 +557,18 @@ def main():
         berr, ferr = errors(torch, at, x, b, x_true)
         res[name] = {"ms": ms, "gmres_iterations": iters, "info": info, "backward_error": berr,
                      "forward_error": ferr, "profile": prof}
+        if prof is not None:
+            for key, (cnt, wk, tms) in prof.items():
+                log(f"{name} {key}: {cnt} launches, {tms:.2f} ms summed")
         log(f"{name}: {ms:.2f} ms, iters {iters}, info {info}, berr {berr:.3e}, ferr {ferr:.3e}")
 
     flops = 2.0 / 3.0 * n ** 3 + 2.0 * n ** 2
     sb = res["SB"]
     value = flops / (sb["ms"] * 1e-3) / 1e9
-    launches, gflop, gms = sb["profile"] if sb["profile"] is not None else (0, 0.0, 0.0)
+    prof = sb["profile"] or {}
+    launches, gflop, gms = prof.get("gemm", (0, 0.0, 0.0))
     achieved = gflop / (gms * 1e-3) if gms > 0 else 0.0
+    leaves, leaf_bytes, leaf_ms = prof.get("leaf", (0, 0.0, 0.0))
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -422,9 +594,17 @@ def main():
                      "frac": round(achieved / PEAK_BF16, 4), "traffic": measured_traffic(n),
                      "launches": launches, "avg_launch_ms": round(gms / max(launches, 1), 4),
                      "avg_launch_gflop": round(gflop / max(launches, 1) / 1e9, 3)},
+        "panel_leaf": {"bound": "latency (sequential columns); HBM shown for scale",
+                       "kernel": "tournament-pivoting panel leaves (tree + apply launches, 32 columns each)",
+                       "launches": leaves, "ms_total": round(leaf_ms, 3),
+                       "avg_launch_ms": round(leaf_ms / max(leaves, 1), 4),
+                       "achieved": round(leaf_bytes / max(leaf_ms * 1e-3, 1e-12) / 1e9, 1), "peak": PEAK_HBM / 1e9,
+                       "unit": "GB/s",
+                       "frac": round(leaf_bytes / max(leaf_ms * 1e-3, 1e-12) / PEAK_HBM, 5)},
         "backward_error": sb["backward_error"],
         "forward_error_vs_fp64_truth": sb["forward_error"],
         "gmres_iterations": sb["gmres_iterations"],
+        "gmres_iterations_by_mode": {k: v["gmres_iterations"] for k, v in res.items()},
     }
     if "SS" in res:
         ss = res["SS"]
@@ -434,10 +614,15 @@ def main():
     if "SH" in res:
         sh = res["SH"]
         out["fp16_ir"] = {"ms": round(sh["ms"], 3), "GFLOP_per_s": round(flops / (sh["ms"] * 1e-3) / 1e9, 1),
-                          "backward_error": sh["backward_error"], "gmres_iterations": sh["gmres_iterations"]}
+                          "backward_error": sh["backward_error"], "forward_error": sh["forward_error"],
+                          "gmres_iterations": sh["gmres_iterations"]}
+        if "SS" in res:
+            out["fp16_ir"]["speedup_vs_fp32"] = round(res["SS"]["ms"] / sh["ms"], 3)
     del at
     torch.cuda.empty_cache()
     if not args.skip_batched:
+        log("plain sgetrf+sgetrs n=512")
+        out["small_dense"] = small_dense(A, torch, h)
         log("batched inverse 65536 x 32x32")
         out["batched_inverse"] = batched_inverse(A, torch, h)
         log("uneven batched getrf+getrs, 20000 systems")
@@ -447,6 +632,15 @@ def main():
     if not args.skip_cpu:
         log(f"host LAPACK baseline n={args.cpu_n}")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        if not args.skip_batched:
+            import numpy as np
+
+            log("host LAPACK baselines of the other workloads")
+            out["small_dense"]["cpu_baseline"] = cpu_baseline(512, reps=51)
+            out["batched_inverse"]["cpu_baseline"] = cpu_batched_inverse_baseline(sample=65536)
+            sizes = np.random.default_rng(20000).integers(8, 257, size=20000)
+            out["uneven_batched"]["cpu_baseline"] = cpu_uneven_baseline(sizes, sample=20000)
+            out["complex"]["cpu_baseline"] = cpu_complex_baseline()
     h.close()
     print(json.dumps(out), flush=True)
     return 0
