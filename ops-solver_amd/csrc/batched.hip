@@ -24,6 +24,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <cstdlib>
 
 #include "internal.h"
 
@@ -180,6 +181,135 @@ __global__ __launch_bounds__(256) void matinv_gj_kernel(int n, const float* cons
     if (live && lane == 0) info[mat] = zero;
 }
 
+// The same Gauss-Jordan inverse with R rows per lane: NL lanes own one matrix (n <= NL*R),
+// lane l holding rows l*R .. l*R+R-1, so a wave carries 64/NL matrices and each pivot step's
+// search, broadcast and reciprocal are shared by R rows of multiply-adds. Pivot keys,
+// scaling and the final scatter are those of matinv_gj_kernel.
+template <int NL, int R>
+__global__ __launch_bounds__(256) void matinv_gj_rows_kernel(int n, const float* const* __restrict__ Aarr,
+                                                             const float* __restrict__ Abase, int lda, int64_t strideA,
+                                                             float* const* __restrict__ Carr, float* __restrict__ Cbase,
+                                                             int ldc, int64_t strideC, int* __restrict__ ipivOut,
+                                                             int* __restrict__ info, int batch)
+{
+    constexpr int N = NL * R;  // columns held per row
+    constexpr int GROUPS = 256 / NL;
+    __shared__ __attribute__((aligned(16))) float s_row[GROUPS][N];
+    __shared__ int s_inv[GROUPS][N];
+    const int tid = threadIdx.x;
+    const int g = tid / NL, lane = tid % NL;
+    const int mat = blockIdx.x * GROUPS + g;
+    const bool live = mat < batch;
+    const float* A = nullptr;
+    float* C = nullptr;
+    if (live)
+    {
+        A = Aarr ? Aarr[mat] : Abase + static_cast<int64_t>(mat) * strideA;
+        C = Carr ? Carr[mat] : Cbase + static_cast<int64_t>(mat) * strideC;
+    }
+    float S[R][N];
+    float sigma[R];
+    int pos[R];
+    bool used[R], rowLive[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+    {
+        const int r = lane * R + i;
+        rowLive[i] = live && r < n;
+#pragma unroll
+        for (int j = 0; j < N; ++j) S[i][j] = (rowLive[i] && j < n) ? A[static_cast<int64_t>(j) * lda + r] : 0.0f;
+        sigma[i] = 1.0f;
+        pos[i] = r;
+        used[i] = false;
+    }
+    int zero = 0;
+
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+    {
+        if (k >= n) continue;
+        unsigned long long best = 0ull;
+        int bi = -1;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+        {
+            const unsigned long long key =
+                (rowLive[i] && !used[i]) ? GjKey(S[i][k], (pos[i] << 8) | (lane * R + i), pos[i] == k) : 0ull;
+            bi = key > best ? i : bi;
+            best = key > best ? key : best;
+        }
+        const unsigned long long top = GroupMaxKey<NL>(best);
+        const int key = static_cast<int>(~static_cast<unsigned>(top & 0xffffffffull));
+        const int p = key & 0xFF;  // pivot row
+        const int ppos = key >> 8;
+        if (live && lane == 0 && ipivOut != nullptr) ipivOut[static_cast<int64_t>(mat) * n + k] = ppos + 1;
+        // LAPACK interchange of positions k and ppos.
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+        {
+            const bool isP = lane * R + i == p;
+            if (pos[i] == k) pos[i] = ppos;
+            if (isP) pos[i] = k;
+        }
+        // Broadcast the pivot row.
+        if (best == top && bi >= 0)
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (i == bi)
+#pragma unroll
+                    for (int j = 0; j < N; ++j) s_row[g][j] = S[i][j];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        float u[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) u[j] = s_row[g][j];
+        const float d = u[k];
+        if (d == 0.0f)
+        {
+            if (zero == 0) zero = k + 1;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (lane * R + i == p) used[i] = true;
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
+        const float rc = 1.0f / d;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+        {
+            const bool isP = lane * R + i == p;
+            const float c = isP ? 0.0f : S[i][k] * rc;
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                if (j != k) S[i][j] = fmaf(-c, u[j], S[i][j]);
+            S[i][k] = isP ? 1.0f : -c;
+            if (isP)
+            {
+                sigma[i] = rc;
+                used[i] = true;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // Result: row r (true values sigma*S) goes to row pos; column j goes to pos^-1(j).
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+        if (rowLive[i]) s_inv[g][pos[i]] = lane * R + i;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < n)
+        {
+            const int col = live ? s_inv[g][j] : 0;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (rowLive[i]) C[static_cast<int64_t>(col) * ldc + pos[i]] = sigma[i] * S[i][j];
+        }
+    if (live && lane == 0) info[mat] = zero;
+}
+
 // Inverse from LU factors and pivots: X = U^-1 L^-1 P, as two substitution sweeps over the
 // rows of P*I with the factor rows held by their lanes.
 // Reference: src/sgetri_batched/sgetri_batched_host.cpp:41-83 (RunInverseBatch) and the
@@ -306,6 +436,30 @@ hipError_t LaunchMatinv(int n, const float* const* Aarr, const float* Abase, int
     return hipGetLastError();
 }
 
+template <int NL, int R>
+hipError_t LaunchMatinvRows(int n, const float* const* Aarr, const float* Abase, int lda, int64_t strideA,
+                            float* const* Carr, float* Cbase, int ldc, int64_t strideC, int* ipiv, int* info, int batch,
+                            hipStream_t stream)
+{
+    constexpr int GROUPS = 256 / NL;
+    const int blocks = (batch + GROUPS - 1) / GROUPS;
+    hipLaunchKernelGGL((matinv_gj_rows_kernel<NL, R>), dim3(blocks), dim3(256), 0, stream, n, Aarr, Abase, lda, strideA,
+                       Carr, Cbase, ldc, strideC, ipiv, info, batch);
+    return hipGetLastError();
+}
+
+// Rows per lane for 17..32 (OPS_SOLVER_MATINV_ROWS = 1, 2 or 4; default 2).
+int MatinvRows()
+{
+    static const int r = []
+    {
+        const char* e = std::getenv("OPS_SOLVER_MATINV_ROWS");
+        const int v = e != nullptr ? std::atoi(e) : 2;
+        return v == 1 || v == 4 ? v : 2;
+    }();
+    return r;
+}
+
 template <int NL>
 hipError_t LaunchGetri(int n, const float* const* Aarr, int lda, const int* piv, float* const* Carr, int ldc,
                        int* info, int batch, hipStream_t stream)
@@ -330,7 +484,16 @@ hipError_t MatinvBatched(int n, const float* const* Aarr, const float* Abase, in
     if (n <= 16)
         return LaunchMatinv<16>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, ipiv, info, batch, stream);
     if (n <= 32)
+    {
+        const int rows = MatinvRows();
+        if (rows == 4)
+            return LaunchMatinvRows<8, 4>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, ipiv, info, batch,
+                                          stream);
+        if (rows == 2)
+            return LaunchMatinvRows<16, 2>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, ipiv, info, batch,
+                                           stream);
         return LaunchMatinv<32>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, ipiv, info, batch, stream);
+    }
     if (n <= 64)
         return LaunchMatinv<64>(n, Aarr, Abase, lda, strideA, Carr, Cbase, ldc, strideC, ipiv, info, batch, stream);
     return hipErrorInvalidValue;
