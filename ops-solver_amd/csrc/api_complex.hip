@@ -83,6 +83,29 @@ hipError_t CTransposeRM(int rows, int cols, const float2* src, int64_t lds, floa
     return hipGetLastError();
 }
 
+// Device-pointer check that remembers the last queried allocation [base, end), so the
+// matrices of a batch that share an allocation need one query, not one each.
+struct DeviceRange
+{
+    uintptr_t base = 0, end = 0;
+    bool Holds(const void* p)
+    {
+        const uintptr_t q = reinterpret_cast<uintptr_t>(p);
+        if (end != 0 && q >= base && q < end) return true;
+        if (!IsDevice(p)) return false;
+        void* b = nullptr;
+        size_t size = 0;
+        if (hipMemGetAddressRange(&b, &size, const_cast<void*>(p)) == hipSuccess)
+        {
+            base = reinterpret_cast<uintptr_t>(b);
+            end = base + size;
+        }
+        else
+            (void)hipGetLastError();
+        return true;
+    }
+};
+
 bool Overlap(const void* a, size_t aBytes, const void* b, size_t bBytes)
 {
     if (aBytes == 0 || bBytes == 0) return false;
@@ -112,10 +135,11 @@ int CheckArrays(aclsolverHandle_t h, int n, const void* aArr, int lda, const voi
         int role;
     };
     std::vector<Span> spans;
+    DeviceRange inRange, outRange;
     for (int b = 0; b < batch; ++b)
     {
-        if (in[b] % alignof(float) || !IsDevice(reinterpret_cast<void*>(in[b]))) return aArg;
-        if (out[b] % alignof(float) || !IsDevice(reinterpret_cast<void*>(out[b]))) return cArg;
+        if (in[b] % alignof(float) || !inRange.Holds(reinterpret_cast<void*>(in[b]))) return aArg;
+        if (out[b] % alignof(float) || !outRange.Holds(reinterpret_cast<void*>(out[b]))) return cArg;
         if (Overlap(reinterpret_cast<void*>(out[b]), cBytes, info, sizeof(int) * batch)) return cArg;
         spans.push_back({in[b], in[b] + aBytes, 0});
         spans.push_back({out[b], out[b] + cBytes, 1});

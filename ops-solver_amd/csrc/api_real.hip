@@ -37,8 +37,17 @@ bool Overlap(const void* a, size_t aBytes, const void* b, size_t bBytes)
 }
 
 // Capacity: the allocation holding p must have at least bytes from p on.
-bool Fits(const void* p, size_t bytes)
+// One queried device allocation [base, end): pointers inside it need no further query.
+struct RangeCache
 {
+    uintptr_t base = 0, end = 0;
+};
+
+bool Fits(const void* p, size_t bytes, RangeCache* cache = nullptr)
+{
+    const uintptr_t q = reinterpret_cast<uintptr_t>(p);
+    if (cache != nullptr && cache->end != 0 && q >= cache->base && q < cache->end && bytes <= cache->end - q)
+        return true;
     if (!IsDevice(p)) return false;
     void* base = nullptr;
     size_t size = 0;
@@ -48,6 +57,11 @@ bool Fits(const void* p, size_t bytes)
         return true;  // range unknown: accept (pool allocations)
     }
     auto off = reinterpret_cast<uintptr_t>(p) - reinterpret_cast<uintptr_t>(base);
+    if (cache != nullptr)
+    {
+        cache->base = reinterpret_cast<uintptr_t>(base);
+        cache->end = cache->base + size;
+    }
     return off <= size && bytes <= size - off;
 }
 
@@ -104,12 +118,14 @@ int CheckBatchArrays(aclsolverHandle_t h, int n, const void* aArr, int lda, cons
     };
     std::vector<Span> spans;
     spans.reserve(static_cast<size_t>(batch) * 2);
+    // Matrices of a batch usually share a few allocations: one query per allocation.
+    RangeCache inCache, outCache;
     for (int b = 0; b < batch; ++b)
     {
         const void* pa = reinterpret_cast<const void*>(in[b]);
         const void* pc = reinterpret_cast<const void*>(out[b]);
-        if (in[b] % alignof(float) != 0 || !Fits(pa, aBytes)) return aArg;
-        if (out[b] % alignof(float) != 0 || !Fits(pc, cBytes)) return cArg;
+        if (in[b] % alignof(float) != 0 || !Fits(pa, aBytes, &inCache)) return aArg;
+        if (out[b] % alignof(float) != 0 || !Fits(pc, cBytes, &outCache)) return cArg;
         if (in[b] != in[0] + b * aBytes || out[b] != out[0] + b * cBytes) *contiguous = false;
         if (Overlap(pc, cBytes, info, sizeof(int) * batch)) return cArg;
         if (piv && Overlap(pc, cBytes, piv, sizeof(int) * pv.size())) return pivArg;
