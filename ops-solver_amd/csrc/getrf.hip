@@ -842,11 +842,13 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     // Look-ahead: the caller's stream s0 carries the critical path (the next block column's
     // update and its panel); the side stream s1 carries the rest of each trailing update.
     const bool two = la != nullptr && la->side != nullptr && npanel > 1;
-    hipStream_t s0 = stream, s1 = two ? la->side : stream;
+    const bool split = two && la->panelStream != nullptr;  // panel path on its own CUs
+    hipStream_t s0 = split ? la->panelStream : stream, s1 = two ? la->side : stream;
     if (two)
     {
-        RET_IF(hipEventRecord(la->fork, s0));
+        RET_IF(hipEventRecord(la->fork, stream));
         RET_IF(hipStreamWaitEvent(s1, la->fork, 0));
+        if (split) RET_IF(hipStreamWaitEvent(s0, la->fork, 0));
     }
 
     // Panel k: factor, then list its interchanges and fold them into perm.
@@ -904,6 +906,11 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     }
     if (two) RET_IF(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
     FoldWaitError(w.err, info, s0);
+    if (split)
+    {
+        RET_IF(hipEventRecord(la->join, s0));
+        RET_IF(hipStreamWaitEvent(stream, la->join, 0));
+    }
     return hipGetLastError();
 }
 

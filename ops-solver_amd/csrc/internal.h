@@ -20,6 +20,11 @@ struct LookAhead
     hipEvent_t fork = nullptr;
     hipEvent_t panel[2] = {nullptr, nullptr};
     hipEvent_t bulk[2] = {nullptr, nullptr};
+    // Optional CU partition: the panel path runs on `panelStream`, restricted to a reserved
+    // set of CUs, and `side` on the others, so the latency-bound panel kernels do not share
+    // CUs with the trailing update; `join` hands the result back to the caller's stream.
+    hipStream_t panelStream = nullptr;
+    hipEvent_t join = nullptr;
 };
 
 // Solver handle: the stream plus per-handle options and a reusable scratch allocation.

@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include <new>
+#include <vector>
 
 #include "internal.h"
 
@@ -231,7 +232,28 @@ const LookAhead* HandleLookAhead(aclsolverHandle_t handle)
     const char* env = std::getenv("OPS_SOLVER_LOOKAHEAD");
     if (env != nullptr && std::atoi(env) == 0) return nullptr;
     LookAhead la;
-    bool ok = hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking) == hipSuccess;
+    // OPS_SOLVER_PANEL_CUS=<k> (0 < k < CUs): reserve about k CUs for the panel path and give
+    // the trailing updates the rest (two CU-masked streams, default priority). The reserved
+    // set, (c / 8 + c) % step == 0, takes the same share of every XCD whether CU numbers run
+    // across the XCDs (c % 8) or through them (c / 32).
+    const char* pcu = std::getenv("OPS_SOLVER_PANEL_CUS");
+    const int panelCUs = pcu != nullptr ? std::atoi(pcu) : 0;
+    bool ok = true;
+    if (panelCUs > 0 && panelCUs < h->numCUs)
+    {
+        const int words = (h->numCUs + 31) / 32, step = h->numCUs / panelCUs;
+        std::vector<uint32_t> pmask(words, 0u), smask(words, 0u);
+        for (int c = 0; c < h->numCUs; ++c)
+        {
+            const bool reserved = (c / 8 + c) % step == 0;
+            (reserved ? pmask : smask)[c / 32] |= 1u << (c % 32);
+        }
+        ok = hipExtStreamCreateWithCUMask(&la.panelStream, static_cast<uint32_t>(words), pmask.data()) == hipSuccess &&
+             hipExtStreamCreateWithCUMask(&la.side, static_cast<uint32_t>(words), smask.data()) == hipSuccess &&
+             hipEventCreateWithFlags(&la.join, hipEventDisableTiming) == hipSuccess;
+    }
+    else
+        ok = hipStreamCreateWithFlags(&la.side, hipStreamNonBlocking) == hipSuccess;
     hipEvent_t* evs[5] = {&la.fork, &la.panel[0], &la.panel[1], &la.bulk[0], &la.bulk[1]};
     for (hipEvent_t* e : evs)
         ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
@@ -240,7 +262,9 @@ const LookAhead* HandleLookAhead(aclsolverHandle_t handle)
         (void)hipGetLastError();
         for (hipEvent_t* e : evs)
             if (*e) (void)hipEventDestroy(*e);
+        if (la.join) (void)hipEventDestroy(la.join);
         if (la.side) (void)hipStreamDestroy(la.side);
+        if (la.panelStream) (void)hipStreamDestroy(la.panelStream);
         return nullptr;
     }
     h->la = la;
@@ -255,10 +279,13 @@ OPS_API aclsolverStatus_t aclsolverDestroy(aclsolverHandle_t handle)
     if (h->la.side != nullptr)
     {
         if (hipStreamSynchronize(h->la.side) != hipSuccess) return ACLSOLVER_STATUS_EXECUTION_FAILED;
-        hipEvent_t evs[5] = {h->la.fork, h->la.panel[0], h->la.panel[1], h->la.bulk[0], h->la.bulk[1]};
+        if (h->la.panelStream != nullptr && hipStreamSynchronize(h->la.panelStream) != hipSuccess)
+            return ACLSOLVER_STATUS_EXECUTION_FAILED;
+        hipEvent_t evs[6] = {h->la.fork, h->la.panel[0], h->la.panel[1], h->la.bulk[0], h->la.bulk[1], h->la.join};
         for (hipEvent_t e : evs)
             if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(h->la.side);
+        if (h->la.panelStream) (void)hipStreamDestroy(h->la.panelStream);
         h->la = LookAhead{};
     }
     if (h->scratch != nullptr)
