@@ -511,6 +511,7 @@ def main():
     ap.add_argument("--modes", default="SS,SH,SB",
                     help="gesv variants to time: SS, SH, SB (SB, the headline, always runs)")
     ap.add_argument("--restart", type=int, default=60)
+    ap.add_argument("--tournament", type=int, default=0, help="SH/SB tall panels: 1 tournament pivoting, 0 partial")
     ap.add_argument("--matrix", choices=["qr", "shifted"], default="qr",
                     help="qr: Q1 diag(s) Q2^T with cond 1e6 (the workload); shifted: N(0,1)/sqrt(n) + 2I")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -539,6 +540,7 @@ def main():
     torch.cuda.synchronize()
     h = A.Handle()
     A.lib().aclsolverSetRefinement(h.h, ctypes.c_double(1e-6), 20, args.restart)
+    A.lib().aclsolverSetTournamentPivoting(h.h, args.tournament)
     lw = ctypes.c_size_t(0)
     assert A.lib().aclsolverSBgesv_bufferSize(h.h, n, 1, ctypes.byref(lw)) == A.SUCCESS
     work = torch.empty(lw.value, dtype=torch.uint8, device="cuda")
@@ -595,7 +597,10 @@ def main():
                      "launches": launches, "avg_launch_ms": round(gms / max(launches, 1), 4),
                      "avg_launch_gflop": round(gflop / max(launches, 1) / 1e9, 3)},
         "panel_leaf": {"bound": "latency (sequential columns); HBM shown for scale",
-                       "kernel": "tournament-pivoting panel leaves (tree + apply launches, 32 columns each)",
+                       "kernel": ("tournament-pivoting panel leaves (tree + apply launches, 32 columns each)"
+                                  if args.tournament else
+                                  "partial-pivoting panel leaves (leaf_multi_kernel / leaf_getf2_kernel, one "
+                                  "launch per leaf of the recursive panel)"),
                        "launches": leaves, "ms_total": round(leaf_ms, 3),
                        "avg_launch_ms": round(leaf_ms / max(leaves, 1), 4),
                        "achieved": round(leaf_bytes / max(leaf_ms * 1e-3, 1e-12) / 1e9, 1), "peak": PEAK_HBM / 1e9,
