@@ -110,6 +110,31 @@ int CheckBatchArrays(aclsolverHandle_t h, int n, const void* aArr, int lda, cons
         return -1;
     const size_t aBytes = static_cast<size_t>(lda) * n * sizeof(float);
     const size_t cBytes = static_cast<size_t>(ldc) * n * sizeof(float);
+    *firstA = reinterpret_cast<const float*>(in[0]);
+    *firstC = reinterpret_cast<float*>(out[0]);
+    // Both arrays stepping through memory one whole matrix at a time (the usual layout):
+    // the per-matrix views are then valid exactly when the two unions are, so one check of
+    // each union replaces the per-matrix checks and the sort. Any failure falls through to
+    // the per-matrix checks, which name the argument.
+    bool seq = in[0] % alignof(float) == 0 && out[0] % alignof(float) == 0;
+    for (int b = 1; seq && b < batch; ++b) seq = in[b] == in[0] + b * aBytes && out[b] == out[0] + b * cBytes;
+    if (seq)
+    {
+        const size_t aAll = aBytes * batch, cAll = cBytes * batch;
+        const void* pa = *firstA;
+        const void* pc = *firstC;
+        bool ok = Fits(pa, aAll) && Fits(pc, cAll) && !Overlap(pc, cAll, pa, aAll) &&
+                  !Overlap(pc, cAll, info, sizeof(int) * batch) &&
+                  !(piv && Overlap(pc, cAll, piv, sizeof(int) * pv.size())) &&
+                  !Overlap(pc, cAll, aArr, sizeof(uintptr_t) * batch) && !Overlap(pc, cAll, cArr, sizeof(uintptr_t) * batch);
+        for (size_t i = 0; ok && i < pv.size(); ++i)
+            ok = pv[i] >= static_cast<int>(i % n) + 1 && pv[i] <= n;
+        if (ok)
+        {
+            *contiguous = true;
+            return 0;
+        }
+    }
     *contiguous = true;
     struct Span
     {
@@ -156,8 +181,6 @@ int CheckBatchArrays(aclsolverHandle_t h, int n, const void* aArr, int lda, cons
             writeEnd = sp.e;
         }
     }
-    *firstA = reinterpret_cast<const float*>(in[0]);
-    *firstC = reinterpret_cast<float*>(out[0]);
     return 0;
 }
 
