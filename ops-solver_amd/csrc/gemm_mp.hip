@@ -121,19 +121,41 @@ __global__ __launch_bounds__(256) void pack_frag_kernel(int rows, int K, const f
     for (int p = 0; p < PARTS; ++p) out[(static_cast<size_t>(f) * PARTS + p) * 64 + l] = part[p];
 }
 
-__device__ inline void TileOfXcd(int tilesM, int tilesN, int* tm, int* tn)
+constexpr int kXcds = 8;     // workgroups are dealt round-robin to the 8 XCDs (each its own L2)
+constexpr int kGroupRows = 8;  // tile rows per L2 group: ~64 concurrent tiles per XCD form 8 x 8
+
+// Tile of this workgroup, or false for a spare workgroup of the padded grid (grid:
+// 8 * TilesPerXcd). With 8 or more tile columns, XCD x = blockIdx % 8 owns the tile columns
+// [x * tilesN / 8, (x + 1) * tilesN / 8) and runs through them in groups of 8 tile rows,
+// column by column, so that the ~64 tiles in flight on one XCD share 8 A strips and about
+// 8 B strips in its L2 (instead of one B strip and every A strip). With fewer columns (a
+// tall update) XCD x takes the x-th eighth of the tiles in column-major order.
+__device__ inline bool TileOfXcd(int tilesM, int tilesN, int* tm, int* tn)
 {
-    // Consecutive workgroups go round-robin to the 8 XCDs; give each XCD a contiguous run
-    // of tiles (column-major tile order) so that neighbouring tiles share L2.
-    const int total = tilesM * tilesN;
-    int id = blockIdx.x;
-    if (total % 8 == 0)
+    const int x = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    if (tilesN < kXcds)
     {
-        const int per = total / 8;
-        id = (id % 8) * per + id / 8;
+        const int total = tilesM * tilesN;
+        const int id = x * total / kXcds + local;
+        if (id >= (x + 1) * total / kXcds) return false;
+        *tm = id % tilesM;
+        *tn = id / tilesM;
+        return true;
     }
-    *tm = id % tilesM;
-    *tn = id / tilesM;
+    const int c0 = x * tilesN / kXcds, c1 = (x + 1) * tilesN / kXcds, cols = c1 - c0;
+    if (local >= tilesM * cols) return false;
+    const int perGroup = kGroupRows * cols;  // tiles of a full group
+    const int grp = local / perGroup, idx = local - grp * perGroup;
+    const int rows = min(kGroupRows, tilesM - grp * kGroupRows);
+    *tm = grp * kGroupRows + idx % rows;
+    *tn = c0 + idx / rows;
+    return true;
+}
+
+// Workgroups per XCD of the padded grid of TileOfXcd.
+inline int TilesPerXcd(int tilesM, int tilesN)
+{
+    return tilesN < kXcds ? (tilesM * tilesN + kXcds - 1) / kXcds : tilesM * ((tilesN + kXcds - 1) / kXcds);
 }
 
 // One workgroup (4 waves) per 128 x 128 tile of C; wave (wm, wn) owns the 64 x 64
@@ -151,7 +173,7 @@ __global__ __launch_bounds__(256) void gemm_packed_kernel(int M, int N, int KB, 
     __shared__ VT sA[2][4][SK][PARTS][64];
     __shared__ VT sB[2][4][SK][PARTS][64];
     int tm, tn;
-    TileOfXcd(tilesM, tilesN, &tm, &tn);
+    if (!TileOfXcd(tilesM, tilesN, &tm, &tn)) return;  // uniform: a spare workgroup
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int rb0 = tm * 4, cb0 = tn * 4;
@@ -315,7 +337,7 @@ hipError_t GemmSubPacked(int M, int N, int K, const float* A, int lda, const flo
     float* sb = sa + static_cast<size_t>(p.rowBlocks) * kFragRows;
     const int fa = p.rowBlocks * p.KB, fb = p.colBlocks * p.KB;
     const int tilesM = p.rowBlocks / 4, tilesN = p.colBlocks / 4;
-    const dim3 grid(tilesM * tilesN);
+    const dim3 grid(kXcds * TilesPerXcd(tilesM, tilesN));
     const int64_t one = 1, la = lda, lb = ldb;
     // A(r, k) = A[k * lda + r]; B^T(c, k) = B[c * ldb + k]
 #define PACK_AND_RUN(ET, P, SA, SB)                                                                                 \
