@@ -180,6 +180,17 @@ extern "C"
                                                      float* const dAarray[], float* const dBarray[], int* dIpiv,
                                                      int* dInfo);
     aclsolverStatus_t aclsolverSgesvVbatched_destroyPlan(aclsolverVbatchedPlan_t plan);
+    /* The schedule: for each 16-wide size bin above 64 (65..80, 81..96, ..., 241..256; 12 bins)
+     * the configuration that solves its matrices, as an id: 1 = 2 rows per lane, 4 waves per
+     * matrix (n <= 128); 2 = 2 rows, 8 waves (n <= 128); 3 = 3 rows, 8 waves (n <= 192);
+     * 4 = 3 rows, 16 waves (n <= 192); 5 = 4 rows, 16 waves (n <= 256). Orders <= 64 always
+     * take one wave per matrix. createPlan takes the autotuned schedule of the device;
+     * createPlanWithSchedule takes the caller's (INVALID_VALUE for an id outside 1..5 or one
+     * whose largest order is below its bin's); getSchedule reports a plan's. */
+    aclsolverStatus_t aclsolverSgesvVbatched_createPlanWithSchedule(aclsolverHandle_t handle, int batchSize,
+                                                                    const int* nArray, int nrhs, const int* schedule,
+                                                                    aclsolverVbatchedPlan_t* plan);
+    aclsolverStatus_t aclsolverSgesvVbatched_getSchedule(aclsolverVbatchedPlan_t plan, int* schedule);
     aclsolverStatus_t aclsolverSgesvVbatched(aclsolverHandle_t handle, int batchSize, const int* nArray, int nrhs,
                                              float* const dAarray[], float* const dBarray[], int* dIpiv, int* dInfo);
 
@@ -193,8 +204,9 @@ extern "C"
     /* ---- EXTENSION (measurement): per-launch timing of instrumented kernels. Between
      * Begin and End, launches of kernel class k are bracketed by HIP events on their own
      * stream; End reports the launch count, summed algorithmic work (flop or byte) and the
-     * summed duration (ms). k = 0: trailing-update GEMM of getrf; 1: batched inverse;
-     * 2: uneven batched getrf+getrs. */
+     * summed duration (ms). k = 0: packed trailing-update GEMM of the mixed-precision getrf
+     * (flop); 1: batched inverse; 2: uneven batched getrf+getrs (byte); 3: panel leaves of
+     * getrf (byte); 4: trailing-update GEMM of cgetrf (real flop). */
     aclsolverStatus_t aclsolverProfileBegin(void);
     aclsolverStatus_t aclsolverProfileEnd(int kernel, int* launches, double* work, double* ms);
 

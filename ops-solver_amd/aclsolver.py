@@ -87,6 +87,8 @@ def lib():
         "aclsolverSgesvVbatched_createPlan": ([P, I, P, I, P], S),
         "aclsolverSgesvVbatched_execute": ([P, P, P, P, P, P], S),
         "aclsolverSgesvVbatched_destroyPlan": ([P], S),
+        "aclsolverSgesvVbatched_createPlanWithSchedule": ([P, I, P, I, P, P], S),
+        "aclsolverSgesvVbatched_getSchedule": ([P, P], S),
         "aclsolverSgesvVbatched": ([P, I, P, I, P, P, P, P], S),
         "aclsolverProfileBegin": ([], S),
         "aclsolverProfileEnd": ([I, P, P, P], S),

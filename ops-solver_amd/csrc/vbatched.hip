@@ -1,15 +1,20 @@
 // Uneven batched LU solve (getrf + getrs per matrix), EXTENSION: the reference has no
 // batched getrf/getrs.
 //
-// Plan: matrices are binned by order into four classes, each with its own kernel so that
-// each class gets the register budget (and occupancy) it needs:
-//   n <= 64:        one wave per matrix, 8 matrices per 512-thread workgroup, a persistent
-//                   grid pulling items from a device counter (longest first);
-//   65..128:        column-cyclic kernel, 4 waves, 2 rows per lane;
-//   129..192:       column-cyclic kernel, 8 waves, 3 rows per lane;
-//   193..256:       column-cyclic kernel, 16 waves, 4 rows per lane.
-// The classes are launched largest first, each class ordered by decreasing n
-// (longest-processing-time order).
+// One persistent launch: a grid of one 1024-thread workgroup per CU pulls items from a
+// device queue (largest orders first: longest-processing-time order). An item is a group of
+// matrices of similar order and one configuration, which splits the workgroup's 16 waves:
+//   W64   n <= 64:  one wave per matrix, 16 matrices per item;
+//   C2x4  n <= 128: column-cyclic, 2 rows per lane, 4 waves per matrix, 4 per item;
+//   C2x8  n <= 128: column-cyclic, 2 rows per lane, 8 waves per matrix, 2 per item;
+//   C3x8  n <= 192: column-cyclic, 3 rows per lane, 8 waves per matrix, 2 per item;
+//   C3x16 n <= 192: column-cyclic, 3 rows per lane, 16 waves, 1 per item;
+//   C4x16 n <= 256: column-cyclic, 4 rows per lane, 16 waves, 1 per item.
+// The configuration of each 16-wide size bin above 64 is chosen per device by an autotuner
+// that times every configuration that fits the bin on one item per CU of synthetic systems
+// of the bin's largest order (a few MB of samples generated on the device) and keeps the
+// one with the least time per matrix; a plan may also be given the schedule explicitly.
+// The last workgroup to leave re-arms the queue, so executing a plan is one kernel launch.
 //
 // Column-cyclic kernel (one matrix per workgroup): lane l of every wave holds rows
 // l, l + 64, ... of the matrix; wave w holds columns w, w + NW, w + 2 NW, ... of the augmented
@@ -23,15 +28,18 @@
 // (look-ahead). Rows keep their registers; the position each occupies is tracked
 // identically in every wave (implicit pivoting, as in the register leaves of the dense
 // factorization). The right-hand sides are extra columns, so the elimination also forward
-// substitutes them; the back substitution is done by one wave from the stored U.
+// substitutes them; the back substitution runs from the stored U, one wave per right-hand
+// side.
 // Outputs: packed LU (rows at their final positions), 1-based ipiv, X, info (first zero
 // pivot).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
-#include <chrono>
 #include <climits>
+#include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -42,13 +50,31 @@ namespace ops
 namespace
 {
 constexpr int kNrhsMax = 4;  // right-hand sides per matrix carried in registers
-constexpr int kWG = 512;
 constexpr int kBS = 32;      // back-substitution block
+
+// Configurations (see above): rows per lane, waves per matrix, matrices per item, largest n.
+enum Cfg : int
+{
+    kW64 = 0,
+    kC2x4 = 1,
+    kC2x8 = 2,
+    kC3x8 = 3,
+    kC3x16 = 4,
+    kC4x16 = 5,
+    kNumCfg = 6
+};
+constexpr int kCfgCap[kNumCfg] = {16, 4, 2, 2, 1, 1};
+constexpr int kCfgMaxN[kNumCfg] = {64, 128, 128, 192, 192, 256};
+constexpr int kBinW = 16;                          // size-bin width above 64
+constexpr int kNumBins = (256 - 64) / kBinW;       // bins 65..80, ..., 241..256
+inline int BinOf(int n) { return (n - 65) / kBinW; }  // n > 64
 
 struct Item
 {
-    int first;  // index into the plan's order of the n <= 64 class
-    int count;  // matrices in this item (<= 8)
+    int first;  // index into the plan's order
+    int count;  // matrices in this item (<= kCfgCap[cfg])
+    int cfg;
+    int pad;
 };
 
 // NaN on the diagonal position (pos == k) wins, NaN elsewhere never does: the reference's
@@ -375,37 +401,6 @@ __device__ __attribute__((always_inline)) void solve_group(int n, int nLoop, int
     if (live && t == 0) *info = zero;
 }
 
-__global__ void reset_counter_kernel(int* counter) { *counter = 0; }
-
-// n <= 64: 8 matrices per workgroup, one wave each (wave-level synchronization only).
-__global__ __launch_bounds__(kWG) void vbatched_kernel(const Item* __restrict__ items, int numItems,
-                                                       const int* __restrict__ order, const int* __restrict__ nArr,
-                                                       const long long* __restrict__ pivOff, int nrhs,
-                                                       float* const* __restrict__ Aarr, float* const* __restrict__ Barr,
-                                                       int* __restrict__ ipiv, int* __restrict__ info,
-                                                       int* __restrict__ counter)
-{
-    __shared__ GroupLds<64, 1> lds[8];
-    __shared__ int s_item;
-    const int tid = threadIdx.x;
-    for (;;)
-    {
-        if (tid == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= numItems) break;  // every workgroup reaches this exit once the queue drains
-        const Item item = items[it];
-        const int g = tid >> 6, t = tid & 63;
-        const bool live = g < item.count;
-        const int mi = live ? order[item.first + g] : 0;
-        const int n = live ? nArr[mi] : 0;
-        solve_group<64, 1, true>(n, n, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
-                                 ipiv + (live ? pivOff[mi] : 0), info + mi, t, live, lds[g]);
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------------------------------
 // Column-cyclic kernel for 65 <= n <= 256.
 
@@ -634,9 +629,9 @@ __device__ __attribute__((always_inline)) inline void PivotColumn(float (&col)[R
 // U (rows 0 .. 64 S + 63, slots 0..S) is read from the stored LU with a prefetch of D columns
 // in registers. x_k = y_k * (1 / U_kk): the reciprocal depends only on U, so it stays off
 // the dependent chain that runs through the y values.
-template <int S, int RPL>
+template <int S, int RPL, int Q>
 __device__ __attribute__((always_inline)) inline void BackSubSeg(const float* __restrict__ A, int n, int lane,
-                                                                 float (&y)[kNrhsMax][RPL])
+                                                                 float (&y)[Q][RPL])
 {
     constexpr int D = 8;
     const int kLo = 64 * S;
@@ -661,7 +656,7 @@ __device__ __attribute__((always_inline)) inline void BackSubSeg(const float* __
                 const int kl = k - kLo;
                 const float r = 1.0f / Rl(u[t][S], kl);
 #pragma unroll
-                for (int q = 0; q < kNrhsMax; ++q)
+                for (int q = 0; q < Q; ++q)
                 {
                     const float x = Rl(y[q][S], kl) * r;
 #pragma unroll
@@ -675,27 +670,39 @@ __device__ __attribute__((always_inline)) inline void BackSubSeg(const float* __
                 u[t][i] = (kn >= kLo && lane + 64 * i < n) ? A[static_cast<size_t>(kn) * n + lane + 64 * i] : 0.0f;
         }
     }
-    if constexpr (S > 0) BackSubSeg<S - 1, RPL>(A, n, lane, y);
+    if constexpr (S > 0) BackSubSeg<S - 1, RPL, Q>(A, n, lane, y);
 }
 
+// Column slots per wave of the column-cyclic solve (a multiple of 4).
 template <int RPL, int NW>
-__global__ __launch_bounds__(64 * NW) void vb_cyclic_kernel(const int* __restrict__ order,
-                                                            const int* __restrict__ nArr,
-                                                            const long long* __restrict__ pivOff, int nrhs,
-                                                            float* const* __restrict__ Aarr,
-                                                            float* const* __restrict__ Barr, int* __restrict__ ipiv,
-                                                            int* __restrict__ info)
+constexpr int CyclicCpw() { return ((64 * RPL + kNrhsMax + NW - 1) / NW + 3) / 4 * 4; }
+template <int RPL, int NW>
+using CyclicLds = CycLds<RPL, NW, CyclicCpw<RPL, NW>()>;
+
+// One matrix (mi; -1: none) by the NW waves with local thread indices tid = 0 .. 64 NW - 1.
+// Every call runs exactly two workgroup barriers, so groups of waves solving different
+// matrices of one item (and groups without a matrix) stay in step.
+template <int RPL, int NW>
+__device__ __attribute__((always_inline)) void CyclicSolve(int mi, int tid, CyclicLds<RPL, NW>& L,
+                                                           const int* __restrict__ nArr,
+                                                           const long long* __restrict__ pivOff, int nrhs,
+                                                           float* const* __restrict__ Aarr,
+                                                           float* const* __restrict__ Barr, int* __restrict__ ipiv,
+                                                           int* __restrict__ info)
 {
-    constexpr int NMAX = 64 * RPL;
-    constexpr int CPW = ((NMAX + kNrhsMax + NW - 1) / NW + 3) / 4 * 4;  // column slots per wave
-    __shared__ CycLds<RPL, NW, CPW> L;
+    constexpr int CPW = CyclicCpw<RPL, NW>();  // column slots per wave
     constexpr int kRing = RingSize<NW>();
-    const int mi = order[blockIdx.x];
+    static_assert(NW >= kNrhsMax, "the back substitution takes one wave per right-hand side");
+    if (mi < 0)
+    {
+        __syncthreads();
+        __syncthreads();
+        return;
+    }
     const int n = nArr[mi];
     float* __restrict__ A = Aarr[mi];
     float* __restrict__ B = Barr[mi];
     int* __restrict__ piv = ipiv + pivOff[mi];
-    const int tid = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     if (tid < kRing) L.tag[tid] = 0ull;
@@ -925,22 +932,117 @@ __global__ __launch_bounds__(64 * NW) void vb_cyclic_kernel(const int* __restric
     if (zero != 0 && lane == 0) atomicMin(&L.zmin, zero);
     __syncthreads();
     if (tid == 0) info[mi] = L.abort ? -1 : (L.zmin == INT_MAX ? 0 : L.zmin);
-    if (w != 0) return;
+    if (w >= nrhs) return;  // NW >= kNrhsMax: one wave per right-hand side
 
-    // Back substitution U x = y by one wave: lane l holds positions l + 64 s. Column k of U
-    // comes from the stored LU; x_k = y_k * (1 / U_kk); rows above k fold x_k in.
-    float y[kNrhsMax][RPL];
+    // Back substitution U x = y of right-hand side w by wave w: lane l holds positions
+    // l + 64 s. Column k of U comes from the stored LU; x_k = y_k * (1 / U_kk); rows above k
+    // fold x_k in.
+    float y[1][RPL];
 #pragma unroll
-    for (int q = 0; q < kNrhsMax; ++q)
+    for (int i = 0; i < RPL; ++i) y[0][i] = lane + 64 * i < n ? L.y[w][lane + 64 * i] : 0.0f;
+    BackSubSeg<RPL - 1, RPL, 1>(A, n, lane, y);
 #pragma unroll
-        for (int i = 0; i < RPL; ++i) y[q][i] = (q < nrhs && lane + 64 * i < n) ? L.y[q][lane + 64 * i] : 0.0f;
-    BackSubSeg<RPL - 1, RPL>(A, n, lane, y);
-#pragma unroll
-    for (int q = 0; q < kNrhsMax; ++q)
-        if (q < nrhs)
-#pragma unroll
-            for (int i = 0; i < RPL; ++i)
-                if (lane + 64 * i < n) B[static_cast<size_t>(q) * n + lane + 64 * i] = y[q][i];
+    for (int i = 0; i < RPL; ++i)
+        if (lane + 64 * i < n) B[static_cast<size_t>(w) * n + lane + 64 * i] = y[0][i];
+}
+
+template <int A_, int B_>
+constexpr int CMax() { return A_ > B_ ? A_ : B_; }
+
+// LDS of one item: the largest over the configurations of (matrices per item) x (LDS per matrix).
+constexpr int kItemLds =
+    CMax<CMax<CMax<16 * static_cast<int>(sizeof(GroupLds<64, 1>)), 4 * static_cast<int>(sizeof(CyclicLds<2, 4>))>(),
+              CMax<2 * static_cast<int>(sizeof(CyclicLds<2, 8>)), 2 * static_cast<int>(sizeof(CyclicLds<3, 8>))>()>(),
+         CMax<static_cast<int>(sizeof(CyclicLds<3, 16>)), static_cast<int>(sizeof(CyclicLds<4, 16>))>()>();
+
+// The waves of an item in groups of NW, one matrix each (order[first + group], when there).
+template <int RPL, int NW>
+__device__ __attribute__((always_inline)) void CyclicItem(const Item& item, int tid, unsigned char* smem,
+                                                          const int* __restrict__ order, const int* __restrict__ nArr,
+                                                          const long long* __restrict__ pivOff, int nrhs,
+                                                          float* const* __restrict__ Aarr,
+                                                          float* const* __restrict__ Barr, int* __restrict__ ipiv,
+                                                          int* __restrict__ info)
+{
+    const int grp = __builtin_amdgcn_readfirstlane(tid / (64 * NW));
+    const int mi = grp < item.count ? order[item.first + grp] : -1;
+    CyclicLds<RPL, NW>* L = reinterpret_cast<CyclicLds<RPL, NW>*>(smem) + grp;
+    CyclicSolve<RPL, NW>(mi, tid - grp * 64 * NW, *L, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info);
+}
+
+// The persistent solve: one 1024-thread workgroup per CU takes items until the queue
+// (counters[0]) runs dry; the last workgroup out (counters[1]) resets both counters, so the
+// next launch of the plan starts from an empty queue with no separate reset.
+__global__ __launch_bounds__(1024) void vb_persistent_kernel(const Item* __restrict__ items, int numItems,
+                                                             const int* __restrict__ order,
+                                                             const int* __restrict__ nArr,
+                                                             const long long* __restrict__ pivOff, int nrhs,
+                                                             float* const* __restrict__ Aarr,
+                                                             float* const* __restrict__ Barr, int* __restrict__ ipiv,
+                                                             int* __restrict__ info, int* counters)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kItemLds];
+    __shared__ int s_item[2];  // by iteration parity: read after one barrier, rewritten after the next
+    const int tid = threadIdx.x;
+    int next = 0;              // thread 0: the item fetched ahead
+    if (tid == 0) next = atomicAdd(&counters[0], 1);
+    for (int iter = 0;; ++iter)
+    {
+        if (tid == 0) s_item[iter & 1] = next;
+        __syncthreads();
+        const int it = s_item[iter & 1];
+        if (it >= numItems) break;  // every workgroup reaches this exit once the queue drains
+        // The following item's index, fetched while this one runs.
+        if (tid == 0) next = atomicAdd(&counters[0], 1);
+        const Item item = items[it];
+        switch (item.cfg)  // uniform
+        {
+        case kW64:
+        {
+            const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
+            const bool live = g < item.count;
+            const int mi = live ? order[item.first + g] : 0;
+            const int n = live ? nArr[mi] : 0;
+            solve_group<64, 1, true>(n, n, nrhs, live ? Aarr[mi] : nullptr, live ? Barr[mi] : nullptr,
+                                     ipiv + (live ? pivOff[mi] : 0), info + mi, tid & 63, live,
+                                     reinterpret_cast<GroupLds<64, 1>*>(smem)[g]);
+            break;
+        }
+        case kC2x4: CyclicItem<2, 4>(item, tid, smem, order, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info); break;
+        case kC2x8: CyclicItem<2, 8>(item, tid, smem, order, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info); break;
+        case kC3x8: CyclicItem<3, 8>(item, tid, smem, order, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info); break;
+        case kC3x16: CyclicItem<3, 16>(item, tid, smem, order, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info); break;
+        default: CyclicItem<4, 16>(item, tid, smem, order, nArr, pivOff, nrhs, Aarr, Barr, ipiv, info); break;
+        }
+    }
+    if (tid == 0 && atomicAdd(&counters[1], 1) == static_cast<int>(gridDim.x) - 1)
+    {
+        __hip_atomic_store(&counters[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&counters[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Synthetic systems for the autotuner: count matrices of order n at A + i * n * n (diagonally
+// dominant, deterministic), right-hand sides of ones, pointer arrays.
+__global__ void tune_fill_kernel(int n, int count, int nrhs, float* A, float* B, float** Ap, float** Bp)
+{
+    const size_t per = static_cast<size_t>(n) * n, total = per * count;
+    for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+    {
+        const int i = static_cast<int>(e / per);
+        const int c = static_cast<int>((e % per) / n), r = static_cast<int>(e % n);
+        const float off = 0.25f * static_cast<float>(((r * 7 + c * 13 + i) % 17) - 8) / (8.0f * n);
+        A[e] = (r == c ? 2.0f : 0.0f) + off;
+    }
+    for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         e < static_cast<size_t>(count) * n * (nrhs > 0 ? nrhs : 1); e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        B[e] = 1.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    {
+        Ap[i] = A + static_cast<size_t>(i) * n * n;
+        Bp[i] = B + static_cast<size_t>(i) * n * (nrhs > 0 ? nrhs : 1);
+    }
 }
 
 }  // namespace
@@ -950,17 +1052,14 @@ struct VbatchedPlan
     int device = 0;
     int batch = 0;
     int nrhs = 0;
-    int numItems = 0;  // items of the n <= 64 class
-    int grid = 0;      // persistent grid of the n <= 64 class
-    int first[3] = {0, 0, 0};  // 193..256, 129..192, 65..128: range in the order
-    int count[3] = {0, 0, 0};
+    int numItems = 0;
+    int grid = 0;                // one workgroup per CU
+    int schedule[kNumBins] = {}; // configuration of each size bin above 64
     Item* dItems = nullptr;
     int* dOrder = nullptr;
     int* dN = nullptr;
     long long* dPivOff = nullptr;
-    int* dCounter = nullptr;
-    hipStream_t side = nullptr;  // the n <= 128 classes run here, beside the larger ones
-    hipEvent_t fork = nullptr, join = nullptr;
+    int* dCounter = nullptr;     // queue head and exit count, zero between launches
     double flops = 0.0;
     double bytes = 0.0;
 };
@@ -972,128 +1071,165 @@ static void FreePlan(VbatchedPlan* p)
     if (p->dN) (void)hipFree(p->dN);
     if (p->dPivOff) (void)hipFree(p->dPivOff);
     if (p->dCounter) (void)hipFree(p->dCounter);
-    if (p->fork) (void)hipEventDestroy(p->fork);
-    if (p->join) (void)hipEventDestroy(p->join);
-    if (p->side) (void)hipStreamDestroy(p->side);
     delete p;
 }
 
-// The 193..256 class runs on the caller's stream; the other classes run on the plan's side
-// stream between a fork and a join event, so that they take CUs as the large class's
-// workgroups drain (its tail) instead of waiting for the whole kernel. The pattern is
-// capturable into a hipGraph from the caller's stream.
-hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
+static hipError_t LaunchItems(const Item* items, int numItems, int grid, const int* order, const int* nArr,
+                              const long long* pivOff, int nrhs, float* const* dA, float* const* dB, int* dIpiv,
+                              int* dInfo, int* counters, hipStream_t s)
 {
-    const bool small = p->count[1] > 0 || p->count[2] > 0 || p->numItems > 0;
-    const bool big = p->count[0] > 0;
-    hipStream_t t = (small && big) ? p->side : s;
-    if (t != s)
-    {
-        hipError_t e = hipEventRecord(p->fork, s);
-        if (e == hipSuccess) e = hipStreamWaitEvent(t, p->fork, 0);
-        if (e != hipSuccess) return e;
-    }
-    if (p->count[0] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<4, 16>), dim3(p->count[0]), dim3(1024), 0, s, p->dOrder + p->first[0], p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
-    if (p->count[1] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<3, 8>), dim3(p->count[1]), dim3(512), 0, t, p->dOrder + p->first[1], p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
-    if (p->count[2] > 0)
-        hipLaunchKernelGGL((vb_cyclic_kernel<2, 4>), dim3(p->count[2]), dim3(256), 0, t, p->dOrder + p->first[2], p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo);
-    if (p->numItems > 0)
-    {
-        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, t, p->dCounter);
-        hipLaunchKernelGGL(vbatched_kernel, dim3(p->grid), dim3(kWG), 0, t, p->dItems, p->numItems, p->dOrder, p->dN,
-                           p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo, p->dCounter);
-    }
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && t != s)
-    {
-        e = hipEventRecord(p->join, t);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s, p->join, 0);
-    }
-    return e;
+    hipLaunchKernelGGL(vb_persistent_kernel, dim3(grid), dim3(1024), 0, s, items, numItems, order, nArr, pivOff, nrhs,
+                       dA, dB, dIpiv, dInfo, counters);
+    return hipGetLastError();
 }
 
-// Autotuner: times each candidate persistent-grid size once on synthetic diagonally
-// dominant systems of the plan's exact sizes and keeps the fastest.
-static hipError_t Autotune(VbatchedPlan* p, const std::vector<int>& ns, hipStream_t s, int numCUs)
+// One launch; capturable into a hipGraph from the caller's stream.
+hipError_t VbatchedLaunch(VbatchedPlan* p, float* const* dA, float* const* dB, int* dIpiv, int* dInfo, hipStream_t s)
 {
-    const int batch = p->batch;
-    std::vector<size_t> offA(batch + 1, 0), offB(batch + 1, 0);
-    for (int i = 0; i < batch; ++i)
+    return LaunchItems(p->dItems, p->numItems, p->grid, p->dOrder, p->dN, p->dPivOff, p->nrhs, dA, dB, dIpiv, dInfo,
+                       p->dCounter, s);
+}
+
+// Items of the plan: matrices in decreasing order of n, each group of up to kCfgCap[cfg]
+// consecutive matrices of one configuration an item (longest-processing-time order).
+static std::vector<Item> BuildItems(const std::vector<int>& ns, const std::vector<int>& order, const int* schedule)
+{
+    std::vector<Item> items;
+    Item open[kNumCfg];
+    for (int c = 0; c < kNumCfg; ++c) open[c] = {0, 0, c, 0};
+    for (int i = 0; i < static_cast<int>(order.size()); ++i)
     {
-        offA[i + 1] = offA[i] + static_cast<size_t>(ns[i]) * ns[i];
-        offB[i + 1] = offB[i] + static_cast<size_t>(ns[i]) * p->nrhs;
+        const int n = ns[order[i]];
+        const int c = n <= 64 ? kW64 : schedule[BinOf(n)];
+        // Items hold consecutive entries of the order: start a new one when this
+        // configuration's open item is full or is not the latest item of its kind.
+        if (open[c].count > 0 && (open[c].count == kCfgCap[c] || open[c].first + open[c].count != i))
+        {
+            items.push_back(open[c]);
+            open[c].count = 0;
+        }
+        if (open[c].count == 0) open[c].first = i;
+        ++open[c].count;
     }
-    std::vector<float> hA(offA[batch]), hB(offB[batch], 1.0f);
-    for (int i = 0; i < batch; ++i)
-        for (int c = 0; c < ns[i]; ++c)
-            for (int r = 0; r < ns[i]; ++r)
-                hA[offA[i] + static_cast<size_t>(c) * ns[i] + r] =
-                    (r == c ? 2.0f : 0.0f) + 0.25f * static_cast<float>(((r * 7 + c * 13 + i) % 17) - 8) / 8.0f /
-                                                 static_cast<float>(ns[i]);
+    for (int c = 0; c < kNumCfg; ++c)
+        if (open[c].count > 0) items.push_back(open[c]);
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.first < y.first; });
+    return items;
+}
+
+// Autotuner (per device, once per process): for each bin above 64 and each configuration
+// that fits it, one launch of numCUs items (one per CU) of synthetic systems of the bin's
+// largest order; the configuration with the least time per matrix wins. Bins whose only
+// fitting configuration is C4x16 are not timed.
+static std::mutex g_tuneMutex;
+static std::map<int, std::array<int, kNumBins>> g_tuned;
+
+static hipError_t Autotune(int device, int numCUs, int nrhs, int* schedule, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_tuneMutex);
+    auto hit = g_tuned.find(device);
+    if (hit != g_tuned.end())
+    {
+        std::copy(hit->second.begin(), hit->second.end(), schedule);
+        return hipSuccess;
+    }
+    size_t maxMats = 0, maxElems = 0;
+    for (int c = 1; c < kNumCfg; ++c)
+    {
+        maxMats = std::max<size_t>(maxMats, static_cast<size_t>(numCUs) * kCfgCap[c]);
+        maxElems = std::max<size_t>(maxElems, static_cast<size_t>(numCUs) * kCfgCap[c] * kCfgMaxN[c] * kCfgMaxN[c]);
+    }
     float *dA = nullptr, *dB = nullptr;
     float **dAp = nullptr, **dBp = nullptr;
-    int *dPiv = nullptr, *dInfo = nullptr;
-    size_t pivTotal = 0;
-    for (int v : ns) pivTotal += v;
-    hipError_t e = hipMalloc(&dA, hA.size() * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dB, hB.size() * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dAp, batch * sizeof(float*));
-    if (e == hipSuccess) e = hipMalloc(&dBp, batch * sizeof(float*));
-    if (e == hipSuccess) e = hipMalloc(&dPiv, pivTotal * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&dInfo, batch * sizeof(int));
-    std::vector<float*> pa(batch), pb(batch);
-    for (int i = 0; i < batch && e == hipSuccess; ++i)
+    int *dPiv = nullptr, *dInfo = nullptr, *dOrder = nullptr, *dN = nullptr, *dCnt = nullptr;
+    long long* dOff = nullptr;
+    Item* dItems = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&dA, maxElems * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dB, maxMats * 256 * kNrhsMax * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dAp, maxMats * sizeof(float*));
+    if (e == hipSuccess) e = hipMalloc(&dBp, maxMats * sizeof(float*));
+    if (e == hipSuccess) e = hipMalloc(&dPiv, maxMats * 256 * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dInfo, maxMats * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dOrder, maxMats * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dN, maxMats * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&dOff, maxMats * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc(&dItems, static_cast<size_t>(numCUs) * sizeof(Item));
+    if (e == hipSuccess) e = hipMalloc(&dCnt, 256);
+    if (e == hipSuccess) e = hipMemset(dCnt, 0, 256);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    std::vector<int> iota(maxMats);
+    for (size_t i = 0; i < maxMats; ++i) iota[i] = static_cast<int>(i);
+    if (e == hipSuccess) e = hipMemcpy(dOrder, iota.data(), maxMats * sizeof(int), hipMemcpyHostToDevice);
+    std::array<int, kNumBins> best{};
+    for (int b = 0; b < kNumBins && e == hipSuccess; ++b)
     {
-        pa[i] = dA + offA[i];
-        pb[i] = dB + offB[i];
-    }
-    if (e == hipSuccess) e = hipMemcpy(dAp, pa.data(), batch * sizeof(float*), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dBp, pb.data(), batch * sizeof(float*), hipMemcpyHostToDevice);
-    double bestT = 1e30;
-    int bestGrid = numCUs;
-    for (int mult : {1, 2, 3, 4})
-    {
-        if (e != hipSuccess) break;
-        p->grid = std::max(1, std::min(p->numItems, numCUs * mult));
-        double t = 0.0;
-        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep)
+        const int n = 64 + kBinW * (b + 1);
+        double bestT = 1e30;
+        best[b] = kC4x16;
+        for (int c = 1; c < kNumCfg && e == hipSuccess; ++c)
         {
-            e = hipMemcpyAsync(dA, hA.data(), hA.size() * sizeof(float), hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            const auto t0 = std::chrono::steady_clock::now();
-            if (e == hipSuccess) e = VbatchedLaunch(p, dAp, dBp, dPiv, dInfo, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            const auto t1 = std::chrono::steady_clock::now();
-            t = std::chrono::duration<double>(t1 - t0).count();  // keep the second (warm) run
-        }
-        if (e == hipSuccess && t < bestT)
-        {
-            bestT = t;
-            bestGrid = p->grid;
+            if (kCfgMaxN[c] < n || (n > 192 && c == kC4x16)) continue;  // does not fit / nothing to compare
+            const int cap = kCfgCap[c], mats = numCUs * cap;
+            std::vector<Item> items(numCUs);
+            for (int i = 0; i < numCUs; ++i) items[i] = {i * cap, cap, c, 0};
+            std::vector<int> nv(mats, n);
+            std::vector<long long> off(mats);
+            for (int i = 0; i < mats; ++i) off[i] = static_cast<long long>(i) * n;
+            e = hipMemcpy(dItems, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(dN, nv.data(), mats * sizeof(int), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(dOff, off.data(), mats * sizeof(long long), hipMemcpyHostToDevice);
+            float t = 1e30f;
+            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep)
+            {
+                hipLaunchKernelGGL(tune_fill_kernel, dim3(1024), dim3(256), 0, s, n, mats, nrhs, dA, dB, dAp, dBp);
+                e = hipEventRecord(e0, s);
+                if (e == hipSuccess)
+                    e = LaunchItems(dItems, numCUs, numCUs, dOrder, dN, dOff, nrhs, dAp, dBp, dPiv, dInfo, dCnt, s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                t = std::min(t, ms);
+            }
+            const double perMatrix = static_cast<double>(t) / cap;
+            if (e == hipSuccess && perMatrix < bestT)
+            {
+                bestT = perMatrix;
+                best[b] = c;
+            }
         }
     }
-    p->grid = bestGrid;
     (void)hipFree(dA);
     (void)hipFree(dB);
     (void)hipFree(dAp);
     (void)hipFree(dBp);
     (void)hipFree(dPiv);
     (void)hipFree(dInfo);
-    return e;
+    (void)hipFree(dOrder);
+    (void)hipFree(dN);
+    (void)hipFree(dOff);
+    (void)hipFree(dItems);
+    (void)hipFree(dCnt);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return e;
+    g_tuned[device] = best;
+    std::copy(best.begin(), best.end(), schedule);
+    return hipSuccess;
 }
 
 }  // namespace ops
 
 using namespace ops;
 
-OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t handle, int batchSize,
-                                                            const int* nArray, int nrhs,
-                                                            aclsolverVbatchedPlan_t* plan)
+namespace
+{
+// schedule: kNumBins configuration ids (1..5) for the bins 65..80, ..., 241..256, or null
+// to take the autotuned schedule.
+aclsolverStatus_t CreatePlan(aclsolverHandle_t handle, int batchSize, const int* nArray, int nrhs,
+                             const int* schedule, aclsolverVbatchedPlan_t* plan)
 {
     if (const aclsolverStatus_t g = CheckHandle(handle)) return g;
     if (plan == nullptr || batchSize < 0 || (batchSize > 0 && nArray == nullptr) || nrhs < 0 || nrhs > kNrhsMax)
@@ -1101,28 +1237,26 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
     std::vector<int> ns(nArray, nArray + batchSize);
     for (int v : ns)
         if (v < 1 || v > 256) return ACLSOLVER_STATUS_INVALID_VALUE;
+    if (schedule != nullptr)
+        for (int b = 0; b < kNumBins; ++b)
+            if (schedule[b] < 1 || schedule[b] >= kNumCfg || kCfgMaxN[schedule[b]] < 64 + kBinW * (b + 1))
+                return ACLSOLVER_STATUS_INVALID_VALUE;
     VbatchedPlan* p = new (std::nothrow) VbatchedPlan();
     if (p == nullptr) return ACLSOLVER_STATUS_ALLOC_FAILED;
     p->batch = batchSize;
     p->nrhs = nrhs;
     (void)hipGetDevice(&p->device);
-    // classes, largest first; within a class, larger orders first (longest-processing-time
-    // order). The n <= 64 class is cut into items of up to 8 matrices.
+    p->grid = H(handle)->numCUs;
+    hipStream_t s = Stream(handle);
+    hipError_t e = hipSuccess;
+    if (schedule != nullptr)
+        std::copy(schedule, schedule + kNumBins, p->schedule);
+    else
+        e = Autotune(p->device, p->grid, kNrhsMax, p->schedule, s);
     std::vector<int> order(batchSize);
     for (int i = 0; i < batchSize; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ns[x] > ns[y]; });
-    std::vector<Item> items;
-    {
-        const int lo[3] = {193, 129, 65};
-        int i = 0;
-        for (int c = 0; c < 3; ++c)
-        {
-            p->first[c] = i;
-            while (i < batchSize && ns[order[i]] >= lo[c]) ++i;
-            p->count[c] = i - p->first[c];
-        }
-        for (; i < batchSize; i += 8) items.push_back({i, std::min(8, batchSize - i)});
-    }
+    const std::vector<Item> items = BuildItems(ns, order, p->schedule);
     std::vector<long long> pivOff(batchSize + 1, 0);
     for (int i = 0; i < batchSize; ++i)
     {
@@ -1132,25 +1266,20 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
         p->bytes += (2.0 * n * n + 2.0 * n * nrhs) * 4.0 + n * 4.0 + 4.0;
     }
     p->numItems = static_cast<int>(items.size());
-    hipStream_t s = Stream(handle);
-    hipError_t e = hipSuccess;
-    if (batchSize > 0)
+    if (batchSize > 0 && e == hipSuccess)
     {
         e = hipMalloc(&p->dItems, std::max<size_t>(1, items.size()) * sizeof(Item));
         if (e == hipSuccess) e = hipMalloc(&p->dOrder, batchSize * sizeof(int));
         if (e == hipSuccess) e = hipMalloc(&p->dN, batchSize * sizeof(int));
         if (e == hipSuccess) e = hipMalloc(&p->dPivOff, batchSize * sizeof(long long));
         if (e == hipSuccess) e = hipMalloc(&p->dCounter, 256);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMemset(p->dCounter, 0, 256);
         if (e == hipSuccess)
             e = hipMemcpy(p->dItems, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(p->dOrder, order.data(), batchSize * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(p->dN, ns.data(), batchSize * sizeof(int), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(p->dPivOff, pivOff.data(), batchSize * sizeof(long long), hipMemcpyHostToDevice);
-        if (e == hipSuccess && p->numItems > 0) e = Autotune(p, ns, s, H(handle)->numCUs);
     }
     if (e != hipSuccess)
     {
@@ -1159,6 +1288,31 @@ OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t ha
         return ACLSOLVER_STATUS_ALLOC_FAILED;
     }
     *plan = p;
+    return ACLSOLVER_STATUS_SUCCESS;
+}
+}  // namespace
+
+OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlan(aclsolverHandle_t handle, int batchSize,
+                                                            const int* nArray, int nrhs,
+                                                            aclsolverVbatchedPlan_t* plan)
+{
+    return CreatePlan(handle, batchSize, nArray, nrhs, nullptr, plan);
+}
+
+OPS_API aclsolverStatus_t aclsolverSgesvVbatched_createPlanWithSchedule(aclsolverHandle_t handle, int batchSize,
+                                                                        const int* nArray, int nrhs,
+                                                                        const int* schedule,
+                                                                        aclsolverVbatchedPlan_t* plan)
+{
+    if (schedule == nullptr) return ACLSOLVER_STATUS_INVALID_VALUE;
+    return CreatePlan(handle, batchSize, nArray, nrhs, schedule, plan);
+}
+
+OPS_API aclsolverStatus_t aclsolverSgesvVbatched_getSchedule(aclsolverVbatchedPlan_t plan, int* schedule)
+{
+    if (plan == nullptr || schedule == nullptr) return ACLSOLVER_STATUS_INVALID_VALUE;
+    const VbatchedPlan* p = static_cast<const VbatchedPlan*>(plan);
+    std::copy(p->schedule, p->schedule + kNumBins, schedule);
     return ACLSOLVER_STATUS_SUCCESS;
 }
 

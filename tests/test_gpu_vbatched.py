@@ -179,3 +179,81 @@ def test_vbatched_class_boundaries(torch, nrhs):
         else:
             x = A.from_device_cm(dB[i], n, nrhs)
             assert O.solve_backward_error(mats[i], x, rhs[i]) <= 10 * n * EPS32, (i, n)
+
+
+def _check_against_oracle(sizes, mats, rhs, dA, dB, hpiv, hinfo, nrhs):
+    off = 0
+    for i, n in enumerate(sizes):
+        lu = A.from_device_cm(dA[i], n, n)
+        x = A.from_device_cm(dB[i], n, nrhs)
+        ref_lu, ref_piv, ref_info, margin = O.sgetrf(mats[i], margins=True)
+        piv = hpiv[off:off + n]
+        off += n
+        assert hinfo[i] == ref_info, (i, n, hinfo[i], ref_info)
+        if ref_info:
+            continue
+        bad = np.nonzero(margin < 1e-4)[0]
+        upto = int(bad[0]) if len(bad) else n
+        np.testing.assert_array_equal(piv[:upto], ref_piv[:upto])
+        assert O.lu_backward_error(mats[i], lu, piv) <= 10 * n * EPS32, (i, n)
+        assert O.solve_backward_error(mats[i], x, rhs[i]) <= 10 * n * EPS32, (i, n)
+
+
+CFG_MAX_N = {1: 128, 2: 128, 3: 192, 4: 192, 5: 256}
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
+def test_vbatched_forced_schedule(torch, cfg):
+    """Each configuration of the persistent launch on every order it can take: the bins it
+    fits use it, the bins above use the 16-wave configuration. Sizes leave partial items
+    (fewer matrices than an item holds) and one singular system."""
+    rng = np.random.default_rng(500 + cfg)
+    top = CFG_MAX_N[cfg]
+    sizes = [int(v) for v in rng.integers(65, top + 1, size=41)] + [1, 33, 64, 65, top, 250]
+    sizes += [int(v) for v in rng.integers(1, 257, size=13)]
+    nrhs = 3
+    mats, rhs = _batch(rng, sizes, nrhs, singular=(5,))
+    dA, dB, pa, pb = _upload(torch, mats, rhs)
+    ipiv = torch.zeros(sum(sizes), dtype=torch.int32, device="cuda")
+    info = torch.full((len(sizes),), -9, dtype=torch.int32, device="cuda")
+    ns = np.array(sizes, np.int32)
+    schedule = np.array([cfg if 64 + 16 * (b + 1) <= top else 5 for b in range(12)], np.int32)
+    with A.Handle() as h:
+        plan = ctypes.c_void_p(None)
+        assert A.lib().aclsolverSgesvVbatched_createPlanWithSchedule(
+            h.h, len(sizes), ns.ctypes.data, nrhs, schedule.ctypes.data, ctypes.byref(plan)) == A.SUCCESS
+        got = np.zeros(12, np.int32)
+        assert A.lib().aclsolverSgesvVbatched_getSchedule(plan, got.ctypes.data) == A.SUCCESS
+        np.testing.assert_array_equal(got, schedule)
+        # two executions in a row: the queue re-arms itself after each launch
+        for rep in range(2):
+            if rep:
+                for t, m in zip(dA, mats):
+                    t.copy_(A.to_device_cm(m))
+                for t, b in zip(dB, rhs):
+                    t.copy_(A.to_device_cm(b))
+                info.fill_(-9)
+            assert A.lib().aclsolverSgesvVbatched_execute(h.h, plan, A.ptr(pa), A.ptr(pb), A.ptr(ipiv),
+                                                          A.ptr(info)) == A.SUCCESS
+            torch.cuda.synchronize()
+            _check_against_oracle(sizes, mats, rhs, dA, dB, ipiv.cpu().numpy(), info.cpu().numpy(), nrhs)
+        assert A.lib().aclsolverSgesvVbatched_destroyPlan(plan) == A.SUCCESS
+
+
+def test_vbatched_autotuned_schedule_and_bad_schedules(torch):
+    """The autotuned schedule fits every bin; schedules naming no configuration, or one too
+    small for its bin, are refused."""
+    ns = np.array([70, 200, 10], np.int32)
+    with A.Handle() as h:
+        plan = ctypes.c_void_p(None)
+        assert A.lib().aclsolverSgesvVbatched_createPlan(h.h, 3, ns.ctypes.data, 1, ctypes.byref(plan)) == A.SUCCESS
+        got = np.zeros(12, np.int32)
+        assert A.lib().aclsolverSgesvVbatched_getSchedule(plan, got.ctypes.data) == A.SUCCESS
+        for b, c in enumerate(got):
+            assert 1 <= c <= 5 and CFG_MAX_N[int(c)] >= 64 + 16 * (b + 1), (b, c)
+        A.lib().aclsolverSgesvVbatched_destroyPlan(plan)
+        for bad in ([0] * 12, [6] * 12, [1] * 12, [5] * 11 + [4]):
+            sched = np.array(bad, np.int32)
+            p2 = ctypes.c_void_p(None)
+            assert A.lib().aclsolverSgesvVbatched_createPlanWithSchedule(
+                h.h, 3, ns.ctypes.data, 1, sched.ctypes.data, ctypes.byref(p2)) == A.INVALID_VALUE
