@@ -381,6 +381,8 @@ def complex_solve(A, torch, h, n=8192, nrhs=16, steps=2):
         a.copy_(a0)
         b.copy_(b0)
         torch.cuda.synchronize()
+        if s == 1:
+            L.aclsolverProfileBegin()
         ev0.record()
         assert L.aclsolverCgetrf(h.h, n, n, A.ptr(a), n, A.ptr(work), A.ptr(ipiv), A.ptr(info)) == A.SUCCESS
         assert L.aclsolverCgetrsDevice(h.h, 0, n, nrhs, A.ptr(a), n, A.ptr(ipiv), A.ptr(b), n, A.ptr(info)) == A.SUCCESS
@@ -390,15 +392,31 @@ def complex_solve(A, torch, h, n=8192, nrhs=16, steps=2):
             times.append(ev0.elapsed_time(ev1))
         log(f"complex n={n} step {s}: {ev0.elapsed_time(ev1):.2f} ms")
     ms = sum(times) / len(times)
-    # backward error in complex128 (column-major A: matrix = a0^T)
+    launches, gwork, gms = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+    L.aclsolverProfileEnd(4, ctypes.byref(launches), ctypes.byref(gwork), ctypes.byref(gms))
+    # backward error in complex128 (column-major A: matrix = a0^T), and the forward error
+    # against an FP64 solve of the same system (torch.linalg.solve in complex128: LU with
+    # partial pivoting, zgetrf + zgetrs)
     a64 = a0.T.to(torch.complex128)
     x64 = b.T.to(torch.complex128)
-    r = b0.T.to(torch.complex128) - a64 @ x64
+    b64 = b0.T.to(torch.complex128)
+    r = b64 - a64 @ x64
     berr = (r.abs().max() / (a64.abs().sum(1).max() * x64.abs().max() + b0.abs().max())).item()
+    xref = torch.linalg.solve(a64, b64)
+    ferr = ((x64 - xref).abs().max() / xref.abs().max()).item()
+    del a64, x64, b64, r, xref
     flops = 8.0 / 3.0 * n ** 3 + 8.0 * n * n * nrhs
+    peak_fp32 = 157.3e12
+    gemm_rate = gwork.value / max(gms.value * 1e-3, 1e-12)
     out = {"workload": f"cgetrf+cgetrs complex64 n={n}, {nrhs} RHS, re/im N(0,1/2)/sqrt(n) + 1.5 I",
            "ms": round(ms, 3), "GFLOP_per_s": round(flops / ms / 1e6, 1), "backward_error": berr,
-           "info": int(info.item())}
+           "forward_error_vs_fp64_solve": ferr, "info": int(info.item()),
+           "trailing_gemm": {"kernel": "cgemm_sub_kernel: complex trailing update from 4 real FP32 MFMA products "
+                                       "(v_mfma_f32_32x32x2f32), FP32 accumulate",
+                             "launches_per_solve": launches.value // max(steps, 1),
+                             "ms_per_solve": round(gms.value / max(steps, 1), 3),
+                             "achieved": round(gemm_rate / 1e12, 2), "peak": peak_fp32 / 1e12, "unit": "TFLOP/s",
+                             "mfma_frac": round(gemm_rate / peak_fp32, 4)}}
     del a, a0, b, b0, work
     torch.cuda.empty_cache()
     # cmatinv_batched 4096 x 64x64
@@ -478,7 +496,10 @@ def uneven_batched(A, torch, h, steps=5):
         if step:
             ms_total += ev0.elapsed_time(ev1)
     ms = ms_total / steps
+    sched = np.zeros(12, np.int32)
+    assert L.aclsolverSgesvVbatched_getSchedule(plan, sched.ctypes.data) == A.SUCCESS
     L.aclsolverSgesvVbatched_destroyPlan(plan)
+    cfg_names = {1: "C2x4", 2: "C2x8", 3: "C3x8", 4: "C3x16", 5: "C4x16"}
     n = sizes.astype(np.float64)
     bytes_ = float(np.sum((2 * n * n + 2 * n * nrhs) * 4 + n * 4 + 4))
     flops = float(np.sum(2.0 / 3.0 * n ** 3 + 2 * n * n * nrhs))
@@ -492,7 +513,9 @@ def uneven_batched(A, torch, h, steps=5):
         r = (Bi - Ai @ Xi).abs().max() / (Ai.abs().sum(1).max() * Xi.abs().max() + Bi.abs().max())
         worst = max(worst, r.item())
     return {"workload": "20000 FP32 systems, n_i ~ U{8..256}, A_i = N(0,1)/sqrt(n_i) + 2I, 4 RHS; getrf+getrs "
-                        "through one plan: n<=64 one wave per system, 65..256 column-cyclic workgroups",
+                        "through one plan, one persistent launch: n<=64 one wave per system, 65..256 column-cyclic "
+                        "wave groups (configuration per 16-wide size bin from the autotuner)",
+            "schedule": {f"{65 + 16 * b}..{80 + 16 * b}": cfg_names.get(int(c), str(int(c))) for b, c in enumerate(sched)},
             "ms": round(ms, 4), "solves_per_s": round(count / ms * 1e3, 1),
             "GB_per_s": round(bytes_ / ms / 1e6, 1), "hbm_frac": round(bytes_ / (ms * 1e-3) / PEAK_HBM, 4),
             "GFLOP_per_s": round(flops / ms / 1e6, 1), "max_backward_error_sample": worst,

@@ -964,9 +964,12 @@ hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, 
             CRET(CTrsmBlock(nb, right, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, 0,
                             a + static_cast<size_t>(panelEnd) * lda + j0, lda, stream));
             if (m - panelEnd > 0)
+            {
+                ProfileScope prof(ProfileKernel::ComplexGemm, 8.0 * (m - panelEnd) * right * nb, stream);
                 CRET(CGemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
                               a + static_cast<size_t>(panelEnd) * lda + j0, lda,
                               a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, stream));
+            }
         }
     }
     if (m > kCLeafMultiRows) FoldWaitError(w.err, info, stream);

@@ -113,7 +113,8 @@ enum class ProfileKernel
     TrailingGemm = 0,  // the packed BF16/FP16 MFMA trailing-update kernel alone (2 M N K flops)
     BatchedInverse = 1,
     Vbatched = 2,
-    PanelLeaf = 3  // one panel leaf of the LU (all its launches); work = 8 * rows * columns bytes
+    PanelLeaf = 3,   // one panel leaf of the LU (all its launches); work = 8 * rows * columns bytes
+    ComplexGemm = 4  // the complex LU's trailing update (cgemm_sub_kernel); work = 8 M N K real flops
 };
 bool ProfilingOn();
 // Brackets the launches issued during its lifetime with HIP events on `stream`.

@@ -246,3 +246,35 @@ def test_cgetrf_invalid_shapes(torch, handle):
     # cmatinv n <= 32 under the reference limits
     assert L.aclsolverCmatinvBatched(handle.h, 33, None, 33, None, 33, A.ptr(info), 1) == A.INVALID_VALUE
     L.aclsolverSetStrictLimits(handle.h, 0)
+
+
+@pytest.mark.parametrize("n,nrhs", [(500, 3), (2100, 2)])
+def test_cgetrf_cgetrs_forward_error_vs_zgetrf(torch, handle, n, nrhs):
+    """The complex64 factor-and-solve against the FP64 oracle (the oracle's zgetrf, solved in
+    complex128): relative forward error within 10 n eps32 times the infinity-norm condition
+    number of A (the C5 distribution: re/im N(0, 1/2)/sqrt(n) + 1.5 I)."""
+    import scipy.linalg as sla
+
+    rng = np.random.default_rng(7000 + n)
+    a = ((rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * np.sqrt(0.5 / n)
+         + 1.5 * np.eye(n)).astype(np.complex64)
+    b = ((rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))) * np.sqrt(0.5)).astype(np.complex64)
+    dA = A.to_device_cm(a)
+    dB = A.to_device_cm(b)
+    lw = ctypes.c_int(0)
+    assert A.lib().aclsolverCgetrf_bufferSize(handle.h, n, n, A.ptr(dA), n, ctypes.byref(lw)) == A.SUCCESS
+    work = torch.empty(max(lw.value, 1), dtype=torch.complex64, device="cuda")
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    info = torch.full((1,), -9, dtype=torch.int32, device="cuda")
+    assert A.lib().aclsolverCgetrf(handle.h, n, n, A.ptr(dA), n, A.ptr(work), A.ptr(ipiv), A.ptr(info)) == A.SUCCESS
+    assert A.lib().aclsolverCgetrsDevice(handle.h, 0, n, nrhs, A.ptr(dA), n, A.ptr(ipiv), A.ptr(dB), n,
+                                         A.ptr(info)) == A.SUCCESS
+    torch.cuda.synchronize()
+    assert info.item() == 0
+    x = A.from_device_cm(dB, n, nrhs).astype(np.complex128)
+    lu64, piv64, zinfo = O.zgetrf(a.astype(np.complex128))
+    assert zinfo == 0
+    x64 = sla.lu_solve((lu64, piv64 - 1), b.astype(np.complex128))
+    kappa = np.linalg.cond(a.astype(np.complex128), np.inf)
+    ferr = np.abs(x - x64).max() / np.abs(x64).max()
+    assert ferr <= 10 * n * EPS32 * kappa, (ferr, kappa)
