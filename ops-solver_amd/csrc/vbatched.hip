@@ -12,7 +12,7 @@
 //   C4x16 n <= 256: column-cyclic, 4 rows per lane, 16 waves, 1 per item.
 // The configuration of each 16-wide size bin above 64 is chosen per device by an autotuner
 // that times every configuration that fits the bin on one item per CU of synthetic systems
-// of the bin's largest order (a few MB of samples generated on the device) and keeps the
+// of the bin's largest order (at most ~80 MB of samples, generated on the device) and keeps the
 // one with the least time per matrix; a plan may also be given the schedule explicitly.
 // The last workgroup to leave re-arms the queue, so executing a plan is one kernel launch.
 //

@@ -58,7 +58,9 @@ def _check_pivots(piv, ref_piv, margin):
 
 @pytest.mark.parametrize("m,n,seed", [(512, 512, 1), (64, 64, 2), (300, 200, 3), (200, 300, 4), (1, 1, 5),
                                       (5, 7, 6), (1000, 1000, 7), (2100, 2100, 8), (33, 1, 9), (4100, 256, 10),
-                                      (8200, 300, 11), (16384, 72, 12)])
+                                      (8200, 300, 11), (16384, 72, 12),
+                                      # thin: m <= 32 < n, n > 128 (the reference's thin kernel shape)
+                                      (16, 300, 13), (32, 1000, 14), (7, 129, 15), (1, 200, 16)])
 def test_sgetrf_parity(torch, handle, m, n, seed):
     rng = np.random.default_rng(seed)
     a = (rng.standard_normal((m, n)) / np.sqrt(n)).astype(np.float32)
