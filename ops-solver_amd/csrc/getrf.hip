@@ -688,6 +688,17 @@ int MultiLeafRows()
 }
 inline bool UseMultiLeaf(int mp) { return MultiLeafRows() > 0 && mp > MultiLeafRows(); }
 
+// Multi-workgroup leaves by cooperative launch (OPS_SOLVER_LEAF_COOP, default 1).
+bool LeafCooperative()
+{
+    static const bool on = []
+    {
+        const char* e = std::getenv("OPS_SOLVER_LEAF_COOP");
+        return e == nullptr || std::atoi(e) != 0;
+    }();
+    return on;
+}
+
 constexpr int kMultiT = 64, kMultiR = 2, kMultiW = 64;  // one wave of 128 rows per workgroup, 64 columns
 
 hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
@@ -705,6 +716,12 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
     int* err = w.err;
     void* args[] = {&A, &lda, &m, &j0, &c0, &wact, &panelEnd, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &slots,
                     &tagBase, &err, &spinLimit};
+    if (!LeafCooperative())
+    {
+        hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW>), dim3(groups), dim3(kMultiT), 0, stream, A, lda, m, j0,
+                           c0, wact, panelEnd, pivoting, newPanel, ipiv, info, rowsrc, slots, tagBase, err, spinLimit);
+        return hipGetLastError();
+    }
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW>),
                                       dim3(groups), dim3(kMultiT), args, 0, stream);
 }
