@@ -139,6 +139,42 @@ __global__ void finite_arrays_kernel(const float* const* arrays, int rows, int c
     if (__any(bad) && (threadIdx.x % warpSize) == 0) atomicOr(flag, 1);
 }
 
+// Small matrices stored contiguously (lda == rows): each workgroup checks 4 matrices at a
+// time with their loads in flight together, 16-byte loads where the matrix allows them.
+__global__ __launch_bounds__(256) void finite_arrays_small_kernel(const float* const* arrays, int per, int batch,
+                                                                  int* flag)
+{
+    constexpr int U = 4;
+    int bad = 0;
+    for (int b0 = blockIdx.x * U; b0 < batch; b0 += gridDim.x * U)
+    {
+        const float* p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) p[u] = b0 + u < batch ? arrays[b0 + u] : nullptr;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            if (p[u] == nullptr) continue;  // uniform
+            if (per % 4 == 0 && (reinterpret_cast<uintptr_t>(p[u]) & 15) == 0)
+            {
+                const float4* q = reinterpret_cast<const float4*>(p[u]);
+                for (int e = threadIdx.x; e < per / 4; e += blockDim.x)
+                {
+                    const float4 v = q[e];
+                    bad |= !(v.x - v.x == 0.0f) | !(v.y - v.y == 0.0f) | !(v.z - v.z == 0.0f) | !(v.w - v.w == 0.0f);
+                }
+            }
+            else
+                for (int e = threadIdx.x; e < per; e += blockDim.x)
+                {
+                    const float v = p[u][e];
+                    bad |= !(v - v == 0.0f);
+                }
+        }
+    }
+    if (__any(bad) && (threadIdx.x % warpSize) == 0) atomicOr(flag, 1);
+}
+
 static bool ReadFlag(int* dflag, int* hflag, hipStream_t stream, bool* ok)
 {
     *hflag = 0;
@@ -171,6 +207,13 @@ bool AllFiniteArrays(aclsolverHandle_t h, const float* const* arrays, int rows, 
     hipStream_t stream = Stream(h);
     int* dflag = H(h)->dflag;
     FillInfo(dflag, 1, 0, stream);
+    if (lda == rows && rows * cols <= 4096)
+    {
+        const int blocks = std::min((batch + 3) / 4, 65536);
+        hipLaunchKernelGGL(finite_arrays_small_kernel, dim3(blocks), dim3(256), 0, stream, arrays, rows * cols, batch,
+                           dflag);
+        return ReadFlag(dflag, H(h)->hflag, stream, ok);
+    }
     int per = (rows * cols + 255) / 256;
     dim3 grid(std::min(per, 64), std::min(batch, 4096));
     hipLaunchKernelGGL(finite_arrays_kernel, grid, dim3(256), 0, stream, arrays, rows, cols, lda, batch, dflag);

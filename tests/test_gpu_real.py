@@ -292,6 +292,40 @@ def test_smatinv_batched(torch, handle, n, batch):
         np.testing.assert_allclose(inv, ref, rtol=0, atol=1e-4 * max(1.0, np.abs(ref).max()))
 
 
+@pytest.mark.parametrize("n,bad,offset", [(32, float("nan"), 0), (32, float("inf"), 0), (32, float("-inf"), 1),
+                                          (5, float("nan"), 0), (7, float("inf"), 3), (64, float("nan"), 2)])
+def test_smatinv_batched_rejects_nonfinite(torch, handle, n, bad, offset):
+    """The drop-in batched inverse scans its input first: one NaN or Inf anywhere in the
+    batch gives INVALID_VALUE with info = -2 for every matrix and leaves the outputs
+    untouched. offset (floats) shifts every matrix off 16-byte alignment, so both the
+    vector and the scalar form of the scan run; orders 5 and 7 are not multiples of 4."""
+    batch = 41
+    rng = np.random.default_rng(n * 10 + offset)
+    per = n * n + offset
+    pool = torch.zeros(batch * per + 4, device="cuda")
+    mats = []
+    for b in range(batch):
+        m = (rng.standard_normal((n, n)) / np.sqrt(n) + 2 * np.eye(n)).astype(np.float32)
+        mats.append(m)
+        pool[b * per + offset: b * per + offset + n * n] = torch.from_numpy(np.ascontiguousarray(m.T).reshape(-1))
+    pool[23 * per + offset + (n * n) // 2 + 1] = bad
+    dC = [torch.full((n, n), 7.0, device="cuda") for _ in range(batch)]
+    pa = torch.tensor([pool.data_ptr() + 4 * (b * per + offset) for b in range(batch)], dtype=torch.int64,
+                      device="cuda")
+    pc = _ptr_array(torch, dC)
+    info = torch.full((batch,), 9, dtype=torch.int32, device="cuda")
+    st = A.lib().aclsolverSmatinvBatched(handle.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
+    torch.cuda.synchronize()
+    assert st == A.INVALID_VALUE
+    assert (info.cpu().numpy() == -2).all()
+    assert all(bool((c == 7.0).all().item()) for c in dC)
+    # the same batch with the bad value removed goes through
+    pool[23 * per + offset + (n * n) // 2 + 1] = 0.5
+    st = A.lib().aclsolverSmatinvBatched(handle.h, n, A.ptr(pa), n, A.ptr(pc), n, A.ptr(info), batch)
+    torch.cuda.synchronize()
+    assert st == A.SUCCESS and (info.cpu().numpy() == 0).all()
+
+
 def test_smatinv_strided_ipiv_and_singular(torch, handle):
     n, batch = 32, 1000
     rng = np.random.default_rng(5)
