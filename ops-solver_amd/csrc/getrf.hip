@@ -96,7 +96,9 @@ inline Work Layout(void* base, int m)
 // parity, so one barrier per column suffices). At the end each row is stored at its final
 // position, which applies the interchanges to the leaf columns.
 // ---------------------------------------------------------------------------------------
-template <int T, int R, int W>
+// FULL: wact == W, so no column of the unrolled loop is skipped at run time (a skip path
+// would make the compiler keep two copies of the row registers alive per column).
+template <int T, int R, int W, bool FULL>
 __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
                                                        int panelEnd, int pivoting, int newPanel,
                                                        int* __restrict__ ipiv, int* __restrict__ info,
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 #pragma unroll
     for (int j = 0; j < W; ++j)
     {
-        if (j < wact)  // uniform
+        if (FULL || j < wact)  // uniform
         {
             const int par = j & 1;
             // Candidate of this thread: its best row (or, without pivoting, the row at j),
@@ -189,22 +191,28 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                     pos[i] = bpos;
             }
             if (tid == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
-            const float d = prow[j];
+            // The pivot is the same in every thread: made uniform, the reciprocal / division
+            // choice is a scalar branch and the division sequence runs only for a subnormal
+            // pivot.
+            const float d = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(prow[j])));
             if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
             // Branch-free update: rows that are not below the pivot take l = 0, for which
-            // the FMA leaves them unchanged (a zero pivot skips the column likewise). d is
-            // the same in every thread, so the reciprocal / division choice is a uniform
-            // branch (the division sequence only runs for a subnormal pivot).
+            // the FMA leaves them unchanged (a zero pivot skips the column likewise).
             const bool recip = fabsf(d) >= FLT_MIN;  // uniform
             const float r = 1.0f / d;
+            float q[R];
+            if (recip)
+#pragma unroll
+                for (int i = 0; i < R; ++i) q[i] = a[i][j] * r;
+            else
+#pragma unroll
+                for (int i = 0; i < R; ++i) q[i] = a[i][j] / d;
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
                 const bool below = pos[i] > j && d != 0.0f;
-                float q = a[i][j] * r;
-                if (!recip) q = a[i][j] / d;  // uniform branch
-                const float l = below ? q : 0.0f;
-                a[i][j] = below ? q : a[i][j];
+                const float l = below ? q[i] : 0.0f;
+                a[i][j] = below ? q[i] : a[i][j];
 #pragma unroll
                 for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, prow[k], a[i][k]);
             }
@@ -670,8 +678,12 @@ template <int T, int R, int W>
 hipError_t LaunchLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
                       int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
-    hipLaunchKernelGGL((leaf_getf2_kernel<T, R, W>), dim3(1), dim3(T), 0, stream, A, lda, m, j0, c0, wact, panelEnd,
-                       pivoting, newPanel, ipiv, info, w.rowsrc);
+    if (wact == W)
+        hipLaunchKernelGGL((leaf_getf2_kernel<T, R, W, true>), dim3(1), dim3(T), 0, stream, A, lda, m, j0, c0, wact,
+                           panelEnd, pivoting, newPanel, ipiv, info, w.rowsrc);
+    else
+        hipLaunchKernelGGL((leaf_getf2_kernel<T, R, W, false>), dim3(1), dim3(T), 0, stream, A, lda, m, j0, c0, wact,
+                           panelEnd, pivoting, newPanel, ipiv, info, w.rowsrc);
     return hipGetLastError();
 }
 
