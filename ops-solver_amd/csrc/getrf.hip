@@ -157,11 +157,17 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
             if (best == wmax && bi >= 0)  // the wave's winner (keys are unique per position)
             {
                 s_key[par][wid] = wmax;
+                // Selected by a bit of a mask, not by i == bi: the compiler would turn the
+                // equality into a[bi][k], a run-time row index that puts a[][] in scratch.
+                const unsigned bsel = 1u << bi;
 #pragma unroll
-                for (int i = 0; i < R; ++i)
-                    if (i == bi)
+                for (int k = 0; k < W; ++k)
+                {
+                    float v = a[0][k];
 #pragma unroll
-                        for (int k = 0; k < W; ++k) s_row[par][wid][k] = a[i][k];
+                    for (int i = 1; i < R; ++i) v = (bsel >> i) & 1u ? a[i][k] : v;
+                    s_row[par][wid][k] = v;
+                }
             }
             if (lane == 0 && wmax == 0ull) s_key[par][wid] = 0ull;
             __syncthreads();
@@ -214,7 +220,8 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
                 const float l = below ? q[i] : 0.0f;
                 a[i][j] = below ? q[i] : a[i][j];
 #pragma unroll
-                for (int k = j + 1; k < W; ++k) a[i][k] = fmaf(-l, prow[k], a[i][k]);
+                for (int k = 1; k < W; ++k)  // constant bounds: unrolled with the j loop
+                    if (k > j) a[i][k] = fmaf(-l, prow[k], a[i][k]);
             }
         }
     }
@@ -316,7 +323,8 @@ __device__ inline void StoreSlot(unsigned long long* p, unsigned tag, unsigned p
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int R, int W>
+// FULL: wact == W (no column of the unrolled loop is skipped at run time).
+template <int R, int W, bool FULL>
 __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
                                                         int panelEnd, int pivoting, int newPanel,
                                                         int* __restrict__ ipiv, int* __restrict__ info,
@@ -364,7 +372,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
 #pragma unroll
     for (int j = 0; j < W; ++j)
     {
-        if (j < wact && !aborted)  // uniform
+        if (FULL || j < wact)  // uniform
         {
             const unsigned tag = tagBase + static_cast<unsigned>(j);
             unsigned long long* base = slots + static_cast<size_t>(j & 1) * kMaxLeafGroups * kSlotWords;
@@ -390,7 +398,8 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 StoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
                 StoreSlot(mine + 1, tag, static_cast<unsigned>(top));
             }
-            // Deferred: the previous pivot on columns j+1 .. W-1.
+            // Deferred: the previous pivot on columns j+1 .. W-1 (constant loop bounds, so
+            // the loop unrolls with the column loop).
             if (j > 0)
 #pragma unroll
                 for (int k = j + 1; k < W; ++k)
@@ -411,7 +420,11 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             // The P keys: lane l reads the two key words of workgroups l and l + 64.
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
-            for (unsigned spins = 0; spins < spinLimit; ++spins)
+            // After a timeout the remaining columns run without waiting (no exchange, no
+            // interchange): the column loop stays one straight unrolled sequence, and the
+            // factors end as NaN.
+            const unsigned lim = aborted ? 0u : spinLimit;
+            for (unsigned spins = 0; spins < lim; ++spins)
             {
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
@@ -455,11 +468,11 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             {
                 if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 aborted = true;
-                continue;
             }
             if (lane < W) s_prow[j & 1][lane] = __uint_as_float(static_cast<unsigned>(rw));
             __syncthreads();
-            const int bpos = gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+            const int bpos =
+                aborted || gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
@@ -473,18 +486,25 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 s_bpos[j] = bpos;
                 if (g == 0 && ipiv != nullptr) ipiv[c0 + j] = c0 + bpos + 1;
             }
-            const float d = s_prow[j & 1][j];
-            if (d == 0.0f && zeroCol == 0) zeroCol = c0 + j + 1;
+            // The pivot, read from lane j of the winner's row into a scalar register: the
+            // reciprocal / division choice is then a scalar branch.
+            const float d = __uint_as_float(__builtin_amdgcn_readlane(static_cast<unsigned>(rw), j));
+            if (d == 0.0f && zeroCol == 0 && !aborted) zeroCol = c0 + j + 1;
             const bool recip = fabsf(d) >= FLT_MIN;  // uniform
             const float r = 1.0f / d;
+            float q[R];
+            if (recip)
+#pragma unroll
+                for (int i = 0; i < R; ++i) q[i] = a[i][j] * r;
+            else
+#pragma unroll
+                for (int i = 0; i < R; ++i) q[i] = a[i][j] / d;
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
                 const bool below = pos[i] > j && d != 0.0f;
-                float q = a[i][j] * r;
-                if (!recip) q = a[i][j] / d;  // uniform branch
-                lprev[i] = below ? q : 0.0f;
-                a[i][j] = below ? q : a[i][j];
+                lprev[i] = below ? q[i] : 0.0f;
+                a[i][j] = below ? q[i] : a[i][j];
             }
             // Column j+1 now (the next candidate search needs it); the rest next column.
             if (j + 1 < W)
@@ -728,14 +748,22 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
     int* err = w.err;
     void* args[] = {&A, &lda, &m, &j0, &c0, &wact, &panelEnd, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &slots,
                     &tagBase, &err, &spinLimit};
+    const bool full = wact == kMultiW;
     if (!LeafCooperative())
     {
-        hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW>), dim3(groups), dim3(kMultiT), 0, stream, A, lda, m, j0,
-                           c0, wact, panelEnd, pivoting, newPanel, ipiv, info, rowsrc, slots, tagBase, err, spinLimit);
+        if (full)
+            hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW, true>), dim3(groups), dim3(kMultiT), 0, stream, A,
+                               lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, rowsrc, slots, tagBase,
+                               err, spinLimit);
+        else
+            hipLaunchKernelGGL((leaf_multi_kernel<kMultiR, kMultiW, false>), dim3(groups), dim3(kMultiT), 0, stream, A,
+                               lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, rowsrc, slots, tagBase,
+                               err, spinLimit);
         return hipGetLastError();
     }
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW>),
-                                      dim3(groups), dim3(kMultiT), args, 0, stream);
+    const void* fn = full ? reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW, true>)
+                          : reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW, false>);
+    return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(kMultiT), args, 0, stream);
 }
 
 // Panels with more active rows than this take tournament leaves when the caller allows
