@@ -42,12 +42,15 @@ constexpr int kMaxRows = 16384;  // largest m the register-resident leaf support
 // one is still being applied); then the global permutation (m ints).
 constexpr int NBMAX = 512;
 constexpr int kMovedInts = 4 * NBMAX + 8;
-// After the ints (8-byte aligned): the exchange slots of the multi-workgroup leaf (two
-// column parities x kMaxLeafGroups workgroups x kSlotWords tagged 64-bit words) and its
-// timeout flag.
-constexpr int kSlotWords = 128;     // per workgroup: 2 key words + up to 64 row words
+// After the ints (8-byte aligned): the exchange slots of the multi-workgroup leaf and its
+// timeout flag. Slots are tagged 64-bit words, per column parity: the workgroups' keys
+// (2 words each, contiguous, so one poll of all keys touches few cache lines), then their
+// candidate rows (kRowWords words each).
+constexpr int kRowWords = 64;        // per workgroup: up to 64 row words
 constexpr int kMaxLeafGroups = 128;  // workgroups of one multi-workgroup leaf
-constexpr size_t kSlotBytes = 2ull * kMaxLeafGroups * kSlotWords * sizeof(unsigned long long);
+constexpr int kKeyWords = 2 * 2 * kMaxLeafGroups;  // both parities
+constexpr size_t kSlotBytes =
+    (static_cast<size_t>(kKeyWords) + 2ull * kMaxLeafGroups * kRowWords) * sizeof(unsigned long long);
 constexpr size_t kLinvPanelBytes = (NBMAX / 64) * 4096 * sizeof(float);  // block inverses of one panel's L11
 struct Work
 {
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                                                         unsigned tagBase, int* err, unsigned spinLimit)
 {
     constexpr int T = 64;
-    static_assert(W + 2 <= kSlotWords && W <= T, "slot layout");
+    static_assert(W <= kRowWords && W <= T, "slot layout");
     constexpr int MOVES = 2 * W;
     __shared__ float s_row[W];
     __shared__ float s_prow[2][W];  // pivot rows of columns j (slot j % 2) and j - 1
@@ -375,8 +378,8 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
         if (FULL || j < wact)  // uniform
         {
             const unsigned tag = tagBase + static_cast<unsigned>(j);
-            unsigned long long* base = slots + static_cast<size_t>(j & 1) * kMaxLeafGroups * kSlotWords;
-            unsigned long long* mine = base + g * kSlotWords;
+            unsigned long long* keys = slots + (j & 1) * 2 * kMaxLeafGroups;
+            unsigned long long* rows = slots + kKeyWords + static_cast<size_t>(j & 1) * kMaxLeafGroups * kRowWords;
             // Column j is up to date; columns beyond it still lack the previous pivot.
             unsigned long long best = 0ull;
             int bi = -1;
@@ -395,11 +398,10 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             const unsigned long long top = WaveMaxU64Split(best);
             if (lane == 0)
             {
-                StoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
-                StoreSlot(mine + 1, tag, static_cast<unsigned>(top));
+                StoreSlot(keys + 2 * g, tag, static_cast<unsigned>(top >> 32));
+                StoreSlot(keys + 2 * g + 1, tag, static_cast<unsigned>(top));
             }
-            // Deferred: the previous pivot on columns j+1 .. W-1 (constant loop bounds, so
-            // the loop unrolls with the column loop).
+            // Deferred: the previous pivot on columns j+1 .. W-1.
             if (j > 0)
 #pragma unroll
                 for (int k = j + 1; k < W; ++k)
@@ -416,7 +418,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
 #pragma unroll
                         for (int k = 0; k < W; ++k) s_row[k] = a[i][k];
             __syncthreads();
-            if (lane < wact) StoreSlot(mine + 2 + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
+            if (lane < wact) StoreSlot(rows + g * kRowWords + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
             // The P keys: lane l reads the two key words of workgroups l and l + 64.
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
@@ -429,13 +431,13 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
                 {
-                    hi0 = LoadSlot(base + lane * kSlotWords);
-                    lo0 = LoadSlot(base + lane * kSlotWords + 1);
+                    hi0 = LoadSlot(keys + 2 * lane);
+                    lo0 = LoadSlot(keys + 2 * lane + 1);
                 }
                 if (lane + 64 < P)
                 {
-                    hi1 = LoadSlot(base + (lane + 64) * kSlotWords);
-                    lo1 = LoadSlot(base + (lane + 64) * kSlotWords + 1);
+                    hi1 = LoadSlot(keys + 2 * (lane + 64));
+                    lo1 = LoadSlot(keys + 2 * (lane + 64) + 1);
                 }
                 const bool good0 =
                     lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             unsigned long long rw = 0ull;
             for (unsigned spins = 0; ok; ++spins)
             {
-                rw = lane < wact ? LoadSlot(base + winner * kSlotWords + 2 + lane)
+                rw = lane < wact ? LoadSlot(rows + winner * kRowWords + lane)
                                  : static_cast<unsigned long long>(tag) << 32;
                 if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
                 if (spins >= spinLimit) ok = false;
