@@ -264,9 +264,14 @@ __global__ __launch_bounds__(128) void cpanel_swap_trsm_kernel(float2* __restric
 // cpanel_swap_trsm_kernel, exactly as cleaf_kernel does. Spins are bounded; on a timeout the
 // workgroup raises *err and fills its rows with NaN.
 // ---------------------------------------------------------------------------------------
-constexpr int kCSlotWords = 64;      // 2 key words + 2 words per complex column (W <= 16)
+// Slots, per column parity: the workgroups' key high words, then their key low words (each
+// array contiguous, so one poll of all keys touches few cache lines), then their candidate
+// rows (kCRowWords tagged words each).
+constexpr int kCRowWords = 32;  // 2 words per complex column (W <= 16)
 constexpr int kCMaxGroups = 128;
-constexpr size_t kCSlotBytes = 2ull * kCMaxGroups * kCSlotWords * sizeof(unsigned long long);
+constexpr int kCKeyWords = 2 * 2 * kCMaxGroups;  // both parities
+constexpr size_t kCSlotBytes =
+    (static_cast<size_t>(kCKeyWords) + 2ull * kCMaxGroups * kCRowWords) * sizeof(unsigned long long);
 
 template <int CTRL>
 __device__ inline unsigned CDppU32(unsigned v)
@@ -310,7 +315,8 @@ __device__ inline void CStoreSlot(unsigned long long* p, unsigned tag, unsigned 
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int R, int W>
+// FULL: wact == W (no column of the unrolled loop is skipped at run time).
+template <int R, int W, bool FULL>
 __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A, int lda, int m, int c0, int wact,
                                                          int pivoting, int newPanel, int* __restrict__ ipiv,
                                                          int* __restrict__ info, int* __restrict__ rowsrc,
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                                                          unsigned tagBase, int* err, unsigned spinLimit)
 {
     constexpr int T = 64;
-    static_assert(2 * W + 2 <= kCSlotWords && 2 * W <= T, "slot layout");
+    static_assert(2 * W <= kCRowWords && 2 * W <= T, "slot layout");
     __shared__ float s_row[2 * W];   // this wave's candidate row (re, im interleaved)
     __shared__ float s_prow[2 * W];  // the pivot row
     __shared__ int s_bpos[W];
@@ -351,11 +357,15 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
 #pragma unroll
     for (int j = 0; j < W; ++j)
     {
-        if (j < wact && !aborted)  // uniform (one wave)
+        if (FULL || j < wact)  // uniform (one wave)
         {
             const unsigned tag = tagBase + static_cast<unsigned>(j);
-            unsigned long long* base = slots + static_cast<size_t>(j & 1) * kCMaxGroups * kCSlotWords;
-            unsigned long long* mine = base + g * kCSlotWords;
+            unsigned long long* keys = slots + (j & 1) * 2 * kCMaxGroups;
+            unsigned long long* rows = slots + kCKeyWords + static_cast<size_t>(j & 1) * kCMaxGroups * kCRowWords;
+            // After a timeout the remaining columns run without waiting (no exchange, no
+            // interchange): the column loop stays one straight unrolled sequence, and the
+            // factors end as NaN.
+            const unsigned lim = aborted ? 0u : spinLimit;
             unsigned long long best = 0ull;
             int bi = -1;
 #pragma unroll
@@ -381,26 +391,26 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                         }
             __syncthreads();
             if (lane < 2 * wact)
-                CStoreSlot(mine + 2 + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
+                CStoreSlot(rows + g * kCRowWords + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
             else if (lane == 2 * wact)
-                CStoreSlot(mine, tag, static_cast<unsigned>(top >> 32));
+                CStoreSlot(keys + g, tag, static_cast<unsigned>(top >> 32));
             else if (lane == 2 * wact + 1)
-                CStoreSlot(mine + 1, tag, static_cast<unsigned>(top));
+                CStoreSlot(keys + kCMaxGroups + g, tag, static_cast<unsigned>(top));
             // the P keys: lane l reads workgroups l and l + 64
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
-            for (unsigned spins = 0; spins < spinLimit; ++spins)
+            for (unsigned spins = 0; spins < lim; ++spins)
             {
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
                 {
-                    hi0 = CLoadSlot(base + lane * kCSlotWords);
-                    lo0 = CLoadSlot(base + lane * kCSlotWords + 1);
+                    hi0 = CLoadSlot(keys + lane);
+                    lo0 = CLoadSlot(keys + kCMaxGroups + lane);
                 }
                 if (lane + 64 < P)
                 {
-                    hi1 = CLoadSlot(base + (lane + 64) * kCSlotWords);
-                    lo1 = CLoadSlot(base + (lane + 64) * kCSlotWords + 1);
+                    hi1 = CLoadSlot(keys + lane + 64);
+                    lo1 = CLoadSlot(keys + kCMaxGroups + lane + 64);
                 }
                 const bool good0 =
                     lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
@@ -423,7 +433,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             unsigned long long rw = 0ull;
             for (unsigned spins = 0; ok; ++spins)
             {
-                rw = lane < 2 * wact ? CLoadSlot(base + winner * kCSlotWords + 2 + lane)
+                rw = lane < 2 * wact ? CLoadSlot(rows + winner * kCRowWords + lane)
                                      : static_cast<unsigned long long>(tag) << 32;
                 if (__all(static_cast<unsigned>(rw >> 32) == tag)) break;
                 if (spins >= spinLimit) ok = false;
@@ -432,11 +442,11 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             {
                 if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 aborted = true;
-                continue;
             }
             if (lane < 2 * W) s_prow[lane] = __uint_as_float(static_cast<unsigned>(rw));
             __syncthreads();
-            const int bpos = gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+            const int bpos =
+                aborted || gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
@@ -452,7 +462,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             }
             const float2 d = make_float2(s_prow[2 * j], s_prow[2 * j + 1]);
             const bool zero = d.x == 0.0f && d.y == 0.0f;
-            if (zero && zeroCol == 0) zeroCol = c0 + j + 1;
+            if (zero && zeroCol == 0 && !aborted) zeroCol = c0 + j + 1;
             const double2 rinv = Recip64(zero ? make_float2(1.0f, 0.0f) : d);
 #pragma unroll
             for (int i = 0; i < R; ++i)
@@ -462,8 +472,8 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                 const float2 l = below ? q : make_float2(0.0f, 0.0f);
                 a[i][j] = below ? q : a[i][j];
 #pragma unroll
-                for (int k = j + 1; k < W; ++k)
-                    a[i][k] = SubMul(a[i][k], l, make_float2(s_prow[2 * k], s_prow[2 * k + 1]));
+                for (int k = 1; k < W; ++k)  // constant bounds: unrolled with the j loop
+                    if (k > j) a[i][k] = SubMul(a[i][k], l, make_float2(s_prow[2 * k], s_prow[2 * k + 1]));
             }
         }
     }
@@ -879,8 +889,9 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
         int* err = w.err;
         void* args[] = {&A, &lda, &m, &c0, &wact, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &leafmap, &slots,
                         &tagBase, &err, &spinLimit};
-        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W>), dim3(groups),
-                                          dim3(64), args, 0, stream);
+        const void* fn = wact == W ? reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W, true>)
+                                   : reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W, false>);
+        return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(64), args, 0, stream);
     }
 #define CLEAF(T, R, W) LaunchCLeaf<T, R, W>(A, lda, m, j0, c0, wact, pivoting, newPanel, ipiv, info, w, stream)
     if (mp <= 256) return CLEAF(256, 1, 16);

@@ -43,9 +43,9 @@ constexpr int kMaxRows = 16384;  // largest m the register-resident leaf support
 constexpr int NBMAX = 512;
 constexpr int kMovedInts = 4 * NBMAX + 8;
 // After the ints (8-byte aligned): the exchange slots of the multi-workgroup leaf and its
-// timeout flag. Slots are tagged 64-bit words, per column parity: the workgroups' keys
-// (2 words each, contiguous, so one poll of all keys touches few cache lines), then their
-// candidate rows (kRowWords words each).
+// timeout flag. Slots are tagged 64-bit words, per column parity: the workgroups' key
+// high words, then their key low words (each array contiguous, so one poll of all keys
+// touches few cache lines), then their candidate rows (kRowWords words each).
 constexpr int kRowWords = 64;        // per workgroup: up to 64 row words
 constexpr int kMaxLeafGroups = 128;  // workgroups of one multi-workgroup leaf
 constexpr int kKeyWords = 2 * 2 * kMaxLeafGroups;  // both parities
@@ -398,8 +398,8 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             const unsigned long long top = WaveMaxU64Split(best);
             if (lane == 0)
             {
-                StoreSlot(keys + 2 * g, tag, static_cast<unsigned>(top >> 32));
-                StoreSlot(keys + 2 * g + 1, tag, static_cast<unsigned>(top));
+                StoreSlot(keys + g, tag, static_cast<unsigned>(top >> 32));
+                StoreSlot(keys + kMaxLeafGroups + g, tag, static_cast<unsigned>(top));
             }
             // Deferred: the previous pivot on columns j+1 .. W-1.
             if (j > 0)
@@ -431,13 +431,13 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                 unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
                 if (lane < P)
                 {
-                    hi0 = LoadSlot(keys + 2 * lane);
-                    lo0 = LoadSlot(keys + 2 * lane + 1);
+                    hi0 = LoadSlot(keys + lane);
+                    lo0 = LoadSlot(keys + kMaxLeafGroups + lane);
                 }
                 if (lane + 64 < P)
                 {
-                    hi1 = LoadSlot(keys + 2 * (lane + 64));
-                    lo1 = LoadSlot(keys + 2 * (lane + 64) + 1);
+                    hi1 = LoadSlot(keys + lane + 64);
+                    lo1 = LoadSlot(keys + kMaxLeafGroups + lane + 64);
                 }
                 const bool good0 =
                     lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
