@@ -43,12 +43,13 @@ constexpr int kMaxRows = 16384;  // largest m the register-resident leaf support
 constexpr int NBMAX = 512;
 constexpr int kMovedInts = 4 * NBMAX + 8;
 // After the ints (8-byte aligned): the exchange slots of the multi-workgroup leaf and its
-// timeout flag. Slots are tagged 64-bit words, per column parity: the workgroups' key
-// high words, then their key low words (each array contiguous, so one poll of all keys
-// touches few cache lines), then their candidate rows (kRowWords words each).
+// timeout flag. Slots are tagged 64-bit words, per column parity: the workgroups' keys
+// (one word each: 16-bit tag, 32-bit magnitude, 16-bit position rank; contiguous, so one
+// poll of all keys touches few cache lines), then their candidate rows (kRowWords words
+// each: 32-bit tag, 32-bit value).
 constexpr int kRowWords = 64;        // per workgroup: up to 64 row words
 constexpr int kMaxLeafGroups = 128;  // workgroups of one multi-workgroup leaf
-constexpr int kKeyWords = 2 * 2 * kMaxLeafGroups;  // both parities
+constexpr int kKeyWords = 2 * kMaxLeafGroups;  // both parities
 constexpr size_t kSlotBytes =
     (static_cast<size_t>(kKeyWords) + 2ull * kMaxLeafGroups * kRowWords) * sizeof(unsigned long long);
 constexpr size_t kLinvPanelBytes = (NBMAX / 64) * 4096 * sizeof(float);  // block inverses of one panel's L11
@@ -378,9 +379,12 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
         if (FULL || j < wact)  // uniform
         {
             const unsigned tag = tagBase + static_cast<unsigned>(j);
-            unsigned long long* keys = slots + (j & 1) * 2 * kMaxLeafGroups;
+            unsigned long long* keys = slots + (j & 1) * kMaxLeafGroups;
             unsigned long long* rows = slots + kKeyWords + static_cast<size_t>(j & 1) * kMaxLeafGroups * kRowWords;
             // Column j is up to date; columns beyond it still lack the previous pivot.
+            // Key: magnitude class in the high word (0: not a candidate, 1: NaN off the
+            // diagonal position, |a| bits + 2, 0x80000000: NaN on it), rank 0xffff - position
+            // (positions are below 16384) in the low word.
             unsigned long long best = 0ull;
             int bi = -1;
 #pragma unroll
@@ -388,19 +392,18 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             {
                 const bool cand = pivoting ? pos[i] >= j : pos[i] == j;
                 const unsigned ab = __float_as_uint(fabsf(a[i][j]));
-                const unsigned mag = ab <= 0x7f800000u ? ab + 1u : (pos[i] == j ? 0x7fffffffu : 0u);
+                const unsigned mag = ab <= 0x7f800000u ? ab + 2u : (pos[i] == j ? 0x80000000u : 1u);
                 const unsigned long long k =
-                    cand ? (1ull << 63) | (static_cast<unsigned long long>(mag) << 32) | static_cast<unsigned>(~pos[i])
-                         : 0ull;
+                    cand ? (static_cast<unsigned long long>(mag) << 32) | static_cast<unsigned>(0xffff - pos[i]) : 0ull;
                 bi = k > best ? i : bi;
                 best = k > best ? k : best;
             }
             const unsigned long long top = WaveMaxU64Split(best);
             if (lane == 0)
-            {
-                StoreSlot(keys + g, tag, static_cast<unsigned>(top >> 32));
-                StoreSlot(keys + kMaxLeafGroups + g, tag, static_cast<unsigned>(top));
-            }
+                __hip_atomic_store(keys + g,
+                                   (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
+                                       (top & 0xffffull),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // Deferred: the previous pivot on columns j+1 .. W-1.
             if (j > 0)
 #pragma unroll
@@ -419,33 +422,22 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
                         for (int k = 0; k < W; ++k) s_row[k] = a[i][k];
             __syncthreads();
             if (lane < wact) StoreSlot(rows + g * kRowWords + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
-            // The P keys: lane l reads the two key words of workgroups l and l + 64.
+            // The P keys: lane l reads the key words of workgroups l and l + 64.
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
             // After a timeout the remaining columns run without waiting (no exchange, no
             // interchange): the column loop stays one straight unrolled sequence, and the
             // factors end as NaN.
             const unsigned lim = aborted ? 0u : spinLimit;
+            const unsigned long long t16 = tag & 0xffffu;
             for (unsigned spins = 0; spins < lim; ++spins)
             {
-                unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
-                if (lane < P)
-                {
-                    hi0 = LoadSlot(keys + lane);
-                    lo0 = LoadSlot(keys + kMaxLeafGroups + lane);
-                }
-                if (lane + 64 < P)
-                {
-                    hi1 = LoadSlot(keys + lane + 64);
-                    lo1 = LoadSlot(keys + kMaxLeafGroups + lane + 64);
-                }
-                const bool good0 =
-                    lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
-                const bool good1 = lane + 64 >= P ||
-                                   (static_cast<unsigned>(hi1 >> 32) == tag && static_cast<unsigned>(lo1 >> 32) == tag);
-                key0 = lane < P ? (hi0 << 32) | (lo0 & 0xffffffffull) : 0ull;
-                key1 = lane + 64 < P ? (hi1 << 32) | (lo1 & 0xffffffffull) : 0ull;
-                if (__all(good0 && good1))
+                const unsigned long long w0 = lane < P ? LoadSlot(keys + lane) : t16 << 48;
+                const unsigned long long w1 = lane + 64 < P ? LoadSlot(keys + lane + 64) : t16 << 48;
+                // (magnitude << 32) | rank, 0 for peers without a candidate
+                key0 = (((w0 >> 16) & 0xffffffffull) << 32) | (w0 & 0xffffull);
+                key1 = (((w1 >> 16) & 0xffffffffull) << 32) | (w1 & 0xffffull);
+                if (__all((w0 >> 48) == t16 && (w1 >> 48) == t16))
                 {
                     ok = true;
                     break;
@@ -473,8 +465,7 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             }
             if (lane < W) s_prow[j & 1][lane] = __uint_as_float(static_cast<unsigned>(rw));
             __syncthreads();
-            const int bpos =
-                aborted || gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+            const int bpos = aborted || gtop == 0ull ? j : 0xffff - static_cast<int>(gtop & 0xffffull);
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
@@ -742,6 +733,9 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
     const int groups = (mp + kMultiT * kMultiR - 1) / (kMultiT * kMultiR);
     if (groups > kMaxLeafGroups || wact > kMultiW || w.epoch == nullptr) return hipErrorInvalidValue;
     unsigned tagBase = (*w.epoch)++ * 64u;
+    // Key words carry 16-bit tags: within one factorization (slots cleared at its start)
+    // the tags must not wrap. At most n / 64 <= 256 leaves (plus tournament epochs) use them.
+    if (tagBase + 64u > 0xffffu) return hipErrorInvalidValue;
     unsigned spinLimit = SpinLimit();
     // Cooperative launch: the workgroups wait for each other every column, so the runtime
     // must make all of them resident at once (it refuses a grid that cannot be).
