@@ -264,12 +264,12 @@ __global__ __launch_bounds__(128) void cpanel_swap_trsm_kernel(float2* __restric
 // cpanel_swap_trsm_kernel, exactly as cleaf_kernel does. Spins are bounded; on a timeout the
 // workgroup raises *err and fills its rows with NaN.
 // ---------------------------------------------------------------------------------------
-// Slots, per column parity: the workgroups' key high words, then their key low words (each
-// array contiguous, so one poll of all keys touches few cache lines), then their candidate
-// rows (kCRowWords tagged words each).
+// Slots, per column parity: the workgroups' keys (one word each: 16-bit tag, 32-bit pivot
+// score, 16-bit position rank; contiguous, so one poll of all keys touches few cache lines),
+// then their candidate rows (kCRowWords words each: 32-bit tag, 32-bit value).
 constexpr int kCRowWords = 32;  // 2 words per complex column (W <= 16)
 constexpr int kCMaxGroups = 128;
-constexpr int kCKeyWords = 2 * 2 * kCMaxGroups;  // both parities
+constexpr int kCKeyWords = 2 * kCMaxGroups;  // both parities
 constexpr size_t kCSlotBytes =
     (static_cast<size_t>(kCKeyWords) + 2ull * kCMaxGroups * kCRowWords) * sizeof(unsigned long long);
 
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
         if (FULL || j < wact)  // uniform (one wave)
         {
             const unsigned tag = tagBase + static_cast<unsigned>(j);
-            unsigned long long* keys = slots + (j & 1) * 2 * kCMaxGroups;
+            unsigned long long* keys = slots + (j & 1) * kCMaxGroups;
             unsigned long long* rows = slots + kCKeyWords + static_cast<size_t>(j & 1) * kCMaxGroups * kCRowWords;
             // After a timeout the remaining columns run without waiting (no exchange, no
             // interchange): the column loop stays one straight unrolled sequence, and the
@@ -371,9 +371,12 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
+                // (score << 32) | (0xffff - position): positions are below 16384, so a
+                // candidate's key is never 0, even with score 0 (NaN)
                 const bool cand = pos[i] != INT_MAX && (pivoting ? pos[i] >= j : pos[i] == j);
                 const unsigned long long k =
-                    cand ? (static_cast<unsigned long long>(PivotScore(a[i][j])) << 32) | static_cast<unsigned>(~pos[i])
+                    cand ? (static_cast<unsigned long long>(PivotScore(a[i][j])) << 32) |
+                               static_cast<unsigned>(0xffff - pos[i])
                          : 0ull;
                 bi = k > best ? i : bi;
                 best = k > best ? k : best;
@@ -393,32 +396,21 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             if (lane < 2 * wact)
                 CStoreSlot(rows + g * kCRowWords + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
             else if (lane == 2 * wact)
-                CStoreSlot(keys + g, tag, static_cast<unsigned>(top >> 32));
-            else if (lane == 2 * wact + 1)
-                CStoreSlot(keys + kCMaxGroups + g, tag, static_cast<unsigned>(top));
+                __hip_atomic_store(keys + g,
+                                   (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
+                                       (top & 0xffffull),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // the P keys: lane l reads workgroups l and l + 64
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
+            const unsigned long long t16 = tag & 0xffffu;
             for (unsigned spins = 0; spins < lim; ++spins)
             {
-                unsigned long long hi0 = 0ull, lo0 = 0ull, hi1 = 0ull, lo1 = 0ull;
-                if (lane < P)
-                {
-                    hi0 = CLoadSlot(keys + lane);
-                    lo0 = CLoadSlot(keys + kCMaxGroups + lane);
-                }
-                if (lane + 64 < P)
-                {
-                    hi1 = CLoadSlot(keys + lane + 64);
-                    lo1 = CLoadSlot(keys + kCMaxGroups + lane + 64);
-                }
-                const bool good0 =
-                    lane >= P || (static_cast<unsigned>(hi0 >> 32) == tag && static_cast<unsigned>(lo0 >> 32) == tag);
-                const bool good1 = lane + 64 >= P ||
-                                   (static_cast<unsigned>(hi1 >> 32) == tag && static_cast<unsigned>(lo1 >> 32) == tag);
-                key0 = lane < P ? (hi0 << 32) | (lo0 & 0xffffffffull) : 0ull;
-                key1 = lane + 64 < P ? (hi1 << 32) | (lo1 & 0xffffffffull) : 0ull;
-                if (__all(good0 && good1))
+                const unsigned long long w0 = lane < P ? CLoadSlot(keys + lane) : t16 << 48;
+                const unsigned long long w1 = lane + 64 < P ? CLoadSlot(keys + lane + 64) : t16 << 48;
+                key0 = (((w0 >> 16) & 0xffffffffull) << 32) | (w0 & 0xffffull);
+                key1 = (((w1 >> 16) & 0xffffffffull) << 32) | (w1 & 0xffffull);
+                if (__all((w0 >> 48) == t16 && (w1 >> 48) == t16))
                 {
                     ok = true;
                     break;
@@ -445,8 +437,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             }
             if (lane < 2 * W) s_prow[lane] = __uint_as_float(static_cast<unsigned>(rw));
             __syncthreads();
-            const int bpos =
-                aborted || gtop == 0ull ? j : static_cast<int>(~static_cast<unsigned>(gtop & 0xffffffffull));
+            const int bpos = aborted || gtop == 0ull ? j : 0xffff - static_cast<int>(gtop & 0xffffull);
 #pragma unroll
             for (int i = 0; i < R; ++i)
             {
@@ -879,7 +870,10 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
         constexpr int R = 2, W = 16;
         const int groups = (mp + 64 * R - 1) / (64 * R);
         if (groups > kCMaxGroups || wact > W || w.epoch == nullptr) return hipErrorInvalidValue;
-        unsigned tagBase = (*w.epoch)++ * 64u;
+        // Tags: W per leaf; key words carry 16 bits of them, which must not wrap within one
+        // factorization (slots cleared at its start): at most n / W leaves.
+        unsigned tagBase = (*w.epoch)++ * static_cast<unsigned>(W);
+        if (tagBase + W > 0xffffu) return hipErrorInvalidValue;
         unsigned spinLimit = SpinLimit();
         // Cooperative launch: the workgroups wait for each other every column, so all of
         // them must be resident at once.
