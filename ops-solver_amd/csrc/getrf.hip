@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cstdlib>
@@ -313,22 +314,39 @@ __global__ __launch_bounds__(T) void leaf_getf2_kernel(float* __restrict__ A, in
 // have published j+1, i.e. after all have finished reading column j. Pivot choice, scaling
 // and update order are those of leaf_getf2_kernel (a NaN on the diagonal position is the
 // pivot, a NaN elsewhere ranks below every number). Spins are bounded:
-// on a timeout the workgroup raises *err, fills its rows with NaN and leaves the column
-// loop; its peers then time out in turn.
+// on a timeout the workgroup raises *err, runs its remaining columns without waiting and
+// stores NaN rows; its peers then time out in turn.
+// SPREAD: the grid holds 8 workgroups per participant and only those with ids = 0 mod 8
+// take part. Workgroups are dispatched to the XCDs round-robin by id, so the participants
+// share one XCD (verified per device by LeafSpreadProbe when a handle is created; without
+// it the leaves run unspread). The exchange words are then stored at workgroup scope --
+// written through the CU's L1 into that XCD's L2 and kept there -- and read with the usual
+// agent-scope loads, which return a line held dirty in the XCD's L2 and fetch clean lines
+// anew: the words never make the trip to the device-coherent memory path (a round trip of
+// ~0.49 us instead of ~0.74 us, measured between two workgroups). Were a participant ever
+// placed on another XCD, it would not see the others' words nor they its: the bounded
+// waits would run out (devInfo -1000), and no stale word passes the tag check.
 // ---------------------------------------------------------------------------------------
 __device__ inline unsigned long long LoadSlot(const unsigned long long* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline void StoreSlot(unsigned long long* p, unsigned tag, unsigned payload)
+__device__ inline void StoreWord(unsigned long long* p, unsigned long long v, bool local)
 {
-    __hip_atomic_store(p, (static_cast<unsigned long long>(tag) << 32) | payload, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if (local)  // uniform
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void StoreSlot(unsigned long long* p, unsigned tag, unsigned payload, bool local = false)
+{
+    StoreWord(p, (static_cast<unsigned long long>(tag) << 32) | payload, local);
 }
 
 // FULL: wact == W (no column of the unrolled loop is skipped at run time).
-template <int R, int W, bool FULL>
+template <int R, int W, bool FULL, bool SPREAD = false>
 __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, int lda, int m, int j0, int c0, int wact,
                                                         int panelEnd, int pivoting, int newPanel,
                                                         int* __restrict__ ipiv, int* __restrict__ info,
@@ -345,12 +363,18 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
     __shared__ int s_mdst[MOVES], s_msrc[MOVES];
 
     const int lane = threadIdx.x;
-    const int P = gridDim.x, g = blockIdx.x;
+    if (SPREAD && (blockIdx.x & 7) != 0) return;  // not a participant
+    const int P = SPREAD ? static_cast<int>(gridDim.x >> 3) : static_cast<int>(gridDim.x);
+    const int g = SPREAD ? static_cast<int>(blockIdx.x >> 3) : static_cast<int>(blockIdx.x);
     const int mp = m - c0;
     float* Ap = A + static_cast<size_t>(c0) * lda + c0;
     const int rbase = g * T * R;
     // An earlier leaf of this factorization timed out: its factors are NaN already.
     if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    bool aborted = false;  // uniform (one wave)
+    // SPREAD: the participants share one XCD (established on the host when the handle was
+    // created), so the exchange words are stored at workgroup scope into its L2.
+    constexpr bool local = SPREAD;
 
     float a[R][W];
     int pos[R], srcv[R];
@@ -368,7 +392,6 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
     }
     if (lane == 0) s_count = 0;
     int zeroCol = 0;
-    bool aborted = false;  // uniform (one wave)
     float lprev[R];  // multipliers of the previous pivot (0 for rows not below it)
 #pragma unroll
     for (int i = 0; i < R; ++i) lprev[i] = 0.0f;
@@ -400,10 +423,10 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
             }
             const unsigned long long top = WaveMaxU64Split(best);
             if (lane == 0)
-                __hip_atomic_store(keys + g,
-                                   (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
-                                       (top & 0xffffull),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                StoreWord(keys + g,
+                          (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
+                              (top & 0xffffull),
+                          local);
             // Deferred: the previous pivot on columns j+1 .. W-1.
             if (j > 0)
 #pragma unroll
@@ -421,7 +444,8 @@ __global__ __launch_bounds__(64) void leaf_multi_kernel(float* __restrict__ A, i
 #pragma unroll
                         for (int k = 0; k < W; ++k) s_row[k] = a[i][k];
             __syncthreads();
-            if (lane < wact) StoreSlot(rows + g * kRowWords + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]));
+            if (lane < wact)
+                StoreSlot(rows + g * kRowWords + lane, tag, __float_as_uint(s_row[lane < W ? lane : 0]), local);
             // The P keys: lane l reads the key words of workgroups l and l + 64.
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
@@ -726,6 +750,28 @@ bool LeafCooperative()
 
 constexpr int kMultiT = 64, kMultiR = 2, kMultiW = 64;  // one wave of 128 rows per workgroup, 64 columns
 
+// Per device: 1 when LeafSpreadProbe found the workgroups with ids = 0 mod 8 of a
+// 1024-workgroup cooperative launch on one XCD, -1 when not, 0 before the probe.
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_spreadOk[kMaxDevices];
+__device__ int g_xccOfGroup[8 * kMaxLeafGroups];
+
+__global__ void xcc_probe_kernel()
+{
+    if (threadIdx.x == 0) g_xccOfGroup[blockIdx.x] = static_cast<int>(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu);
+}
+
+// Full-width tall leaves run spread over 8x the grid and exchange through one XCD's L2 when
+// the probe allowed it; OPS_SOLVER_LEAF_XCD=0 turns that off (read per launch).
+bool LeafSpread()
+{
+    const char* e = std::getenv("OPS_SOLVER_LEAF_XCD");
+    if (e != nullptr && std::atoi(e) == 0) return false;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return false;
+    return g_spreadOk[device].load(std::memory_order_relaxed) == 1;
+}
+
 hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
                      int* ipiv, int* info, const Work& w, hipStream_t stream)
 {
@@ -757,6 +803,9 @@ hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int pan
                                err, spinLimit);
         return hipGetLastError();
     }
+    if (full && LeafSpread())
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW, true, true>),
+                                          dim3(8 * groups), dim3(kMultiT), args, 0, stream);
     const void* fn = full ? reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW, true>)
                           : reinterpret_cast<const void*>(&leaf_multi_kernel<kMultiR, kMultiW, false>);
     return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(kMultiT), args, 0, stream);
@@ -1164,6 +1213,30 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
         RET_IF(PermuteRows(false, n, nrhs, p, x, n, b, ldb, stream));
     }
     return hipGetLastError();
+}
+
+// Records the XCD of every workgroup of a 1024-workgroup cooperative launch and allows the
+// spread tall leaves on this device when those with ids = 0 mod 8 share one XCD. Runs once
+// per device (the device current), synchronously on the null stream.
+void LeafSpreadProbe(int device)
+{
+    if (device < 0 || device >= kMaxDevices || g_spreadOk[device].load() != 0) return;
+    constexpr int G = 8 * kMaxLeafGroups;
+    int result = -1;
+    if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&xcc_probe_kernel), dim3(G), dim3(64), nullptr, 0,
+                                   nullptr) == hipSuccess &&
+        hipStreamSynchronize(nullptr) == hipSuccess)
+    {
+        static int h[G];
+        if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_xccOfGroup), sizeof(h)) == hipSuccess)
+        {
+            result = 1;
+            for (int k = 1; k < kMaxLeafGroups; ++k)
+                if (h[8 * k] != h[0]) result = -1;
+        }
+    }
+    (void)hipGetLastError();
+    g_spreadOk[device].store(result);
 }
 
 }  // namespace ops

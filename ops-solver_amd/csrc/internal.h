@@ -92,6 +92,9 @@ void* HandleScratch(aclsolverHandle_t h, size_t bytes);
 // Device address of this device's fixed no-allocation scratch array (noalloc.hip), looked
 // up once per device with the device current; nullptr when the lookup fails.
 void* NoAllocScratch(int device);
+// Once per device at handle creation: may the tall panel leaves exchange through one XCD's
+// L2 (getrf.hip)?
+void LeafSpreadProbe(int device);
 
 // Writes value into count ints of a Device info array on stream.
 void FillInfo(int* info, int count, int value, hipStream_t stream);

@@ -259,6 +259,8 @@ OPS_API aclsolverStatus_t aclsolverCreate(aclsolverHandle_t* handle)
     // Device address of the no-allocation entry point's scratch array, looked up once per
     // device while it is current (failure here only disables that entry point).
     if (h->archOk) (void)NoAllocScratch(device);
+    // Placement of the tall panel leaves' workgroups (one probe launch per device).
+    if (h->archOk) LeafSpreadProbe(device);
     *handle = h;
     return ACLSOLVER_STATUS_SUCCESS;
 }

@@ -202,6 +202,27 @@ def test_determinism(torch, handle):
         assert np.array_equal(piv, outs[0][1]) and info == outs[0][2]
 
 
+@pytest.mark.parametrize("m,n,seed", [(6000, 200, 5), (3000, 3000, 6)])
+def test_tall_leaf_exchange_paths_agree(torch, handle, m, n, seed, monkeypatch):
+    """The tall leaves' two exchange paths (spread over one XCD's L2, the default where the
+    device allows it, and the device-coherent path, OPS_SOLVER_LEAF_XCD=0) give identical
+    factors; both match the oracle's pivots and meet the backward-error bar."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((m, n)).astype(np.float32)
+    st, lu, piv, info, _ = _sgetrf(torch, handle, a)
+    monkeypatch.setenv("OPS_SOLVER_LEAF_XCD", "0")
+    st0, lu0, piv0, info0, _ = _sgetrf(torch, handle, a)
+    monkeypatch.delenv("OPS_SOLVER_LEAF_XCD")
+    assert st == st0 == A.SUCCESS and info == info0 == 0
+    assert np.array_equal(lu.view(np.uint32), lu0.view(np.uint32))
+    assert np.array_equal(piv, piv0)
+    # the first kp pivots depend only on the first kp columns
+    kp = min(m, n, 256)
+    _, ref_piv, _, margin = O.sgetrf(np.ascontiguousarray(a[:, :kp]), margins=True)
+    _check_pivots(piv[:kp], ref_piv, margin)
+    assert O.lu_backward_error(a, lu, piv) <= 10 * max(m, n) * EPS32
+
+
 def test_invalid_arguments(torch, handle):
     L = A.lib()
     info = torch.full((1,), 5, dtype=torch.int32, device="cuda")
