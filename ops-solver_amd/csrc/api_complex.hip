@@ -258,7 +258,8 @@ OPS_API aclsolverStatus_t aclsolverCgetrf(aclsolverHandle_t handle, int m, int n
     hipStream_t s = Stream(handle);
     FillInfo(devInfo, 1, 0, s);
     if (m == 0 || n == 0) return Launched();
-    if (ComplexGetrf(m, n, reinterpret_cast<float2*>(A), lda, devIpiv, devInfo, Workspace, s) != hipSuccess)
+    if (ComplexGetrf(m, n, reinterpret_cast<float2*>(A), lda, devIpiv, devInfo, Workspace, s,
+                     HandleLookAhead(handle)) != hipSuccess)
         return ACLSOLVER_STATUS_EXECUTION_FAILED;
     return Launched();
 }

@@ -556,14 +556,16 @@ __global__ __launch_bounds__(1024) void ccompact_moves_kernel(int m, int j0, con
 
 constexpr int CNB = 128;  // complex outer block
 
-__global__ __launch_bounds__(256) void couter_swap_kernel(float2* __restrict__ A, int lda, int j0, int nb, int n,
-                                                          const int* __restrict__ moved)
+// Applies a panel's moved-row list to the columns [lo1, hi1) and [lo2, hi2): one wave64 per
+// column gathers every source row, then scatters.
+__global__ __launch_bounds__(256) void couter_swap_kernel(float2* __restrict__ A, int lda, int lo1, int hi1, int lo2,
+                                                          int hi2, const int* __restrict__ moved)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int idx = blockIdx.x * 4 + wave;
-    const int outside = j0 + (n - j0 - nb);
-    if (idx >= outside) return;
-    const int c = idx < j0 ? idx : idx + nb;
+    const int n1 = hi1 - lo1;
+    if (idx >= n1 + (hi2 - lo2)) return;
+    const int c = idx < n1 ? lo1 + idx : lo2 + (idx - n1);
     float2* col = A + static_cast<size_t>(c) * lda;
     const int cnt = moved[0];
     constexpr int MAXQ = (2 * CNB + 63) / 64;
@@ -814,27 +816,30 @@ hipError_t CTrsmBlock(int K, int N, const float2* T, int ldt, bool lower, bool u
     return hipSuccess;
 }
 
-// Workspace: ints [0, m) row sources, the panel's moved-row list (4 CNB + 8), the leaf's
-// map (136); then, 8-byte aligned, the exchange slots of the multi-workgroup leaf and its
-// timeout flag.
+// Workspace: ints [0, m) row sources, two moved-row lists of 4 CNB + 8 (panel k uses list
+// k % 2, so the look-ahead panel can build its list while the previous one is still being
+// applied), the leaf's map (136); then, 8-byte aligned, the exchange slots of the
+// multi-workgroup leaf and its timeout flag.
 constexpr int kCLeafMultiRows = 2048;  // panels taller than this spread their leaves
+constexpr int kCMovedInts = 4 * CNB + 8;
 struct CWork
 {
     int* rowsrc;
-    int* moved;
+    int* moved[2];
     int* leafmap;
     unsigned long long* slots;
     int* err;
     unsigned* epoch;  // host counter: tag base of the next multi-workgroup leaf
 };
-inline size_t CWorkInts(int m) { return static_cast<size_t>(m) + 4 * CNB + 8 + 1 + 2 * 64 + 7; }
+inline size_t CWorkInts(int m) { return static_cast<size_t>(m) + 2 * kCMovedInts + 1 + 2 * 64 + 7; }
 CWork CLayout(void* base, int m)
 {
     int* p = static_cast<int*>(base);
     CWork w;
     w.rowsrc = p;
-    w.moved = p + m;
-    w.leafmap = p + m + 4 * CNB + 8;
+    w.moved[0] = p + m;
+    w.moved[1] = w.moved[0] + kCMovedInts;
+    w.leafmap = w.moved[1] + kCMovedInts;
     uintptr_t q = reinterpret_cast<uintptr_t>(p + CWorkInts(m));
     q = (q + 7) & ~static_cast<uintptr_t>(7);
     w.slots = reinterpret_cast<unsigned long long*>(q);
@@ -913,7 +918,8 @@ size_t ComplexGetrfWorkspaceBytes(int m, int n)
     return CWorkInts(std::max(m, 1)) * sizeof(int) + 8 + kCSlotBytes + 16;
 }
 
-hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream)
+hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream,
+                        const LookAhead* la)
 {
     if (m > 16384) return hipErrorInvalidValue;
     CWork w = CLayout(work, m);
@@ -926,58 +932,94 @@ hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, 
         hipLaunchKernelGGL(cclear_slots_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, w.slots, words, w.err);
     }
     const int kmax = std::min(m, n);
+    if (kmax == 0) return hipGetLastError();
     const int pivoting = ipiv != nullptr;
-    for (int j0 = 0; j0 < kmax; j0 += CNB)
+    const int npanel = (kmax + CNB - 1) / CNB;
+
+    // Look-ahead, as in the real factorization: the caller's stream s0 carries the critical
+    // path (the next block column's update and its panel), the side stream s1 the rest of
+    // each trailing update.
+    const bool two = la != nullptr && la->side != nullptr && npanel > 1;
+    hipStream_t s0 = stream, s1 = two ? la->side : stream;
+    if (two)
     {
-        const int nb = std::min(CNB, kmax - j0);
-        const int panelEnd = j0 + nb;
+        CRET(hipEventRecord(la->fork, stream));
+        CRET(hipStreamWaitEvent(s1, la->fork, 0));
+    }
+
+    // Panel k: its leaves (each followed by the update of the panel's remaining columns),
+    // then the list of its interchanges.
+    auto panel = [&](int k) -> hipError_t
+    {
+        const int j0 = k * CNB, nb = std::min(CNB, kmax - j0), panelEnd = j0 + nb;
         for (int c0 = j0; c0 < panelEnd;)
         {
             const int W = CLeafWidth(m - c0);
             const int wact = std::min(W, panelEnd - c0);
-            CRET(CLeaf(a, lda, m, j0, c0, wact, pivoting, c0 == j0, ipiv, info, w, stream));
+            CRET(CLeaf(a, lda, m, j0, c0, wact, pivoting, c0 == j0, ipiv, info, w, s0));
             const int others = nb - wact;
             if (others > 0)
             {
                 if (pivoting)
-                    hipLaunchKernelGGL(cpanel_swap_trsm_kernel, dim3(others), dim3(128), 0, stream, a, lda, j0, c0,
-                                       wact, w.leafmap);
+                    hipLaunchKernelGGL(cpanel_swap_trsm_kernel, dim3(others), dim3(128), 0, s0, a, lda, j0, c0, wact,
+                                       w.leafmap);
                 else if (panelEnd - c0 - wact > 0)
                     CRET(CTrsmBlock(wact, panelEnd - c0 - wact, a + static_cast<size_t>(c0) * lda + c0, lda, true,
-                                    true, 0, a + static_cast<size_t>(c0 + wact) * lda + c0, lda, stream));
+                                    true, 0, a + static_cast<size_t>(c0 + wact) * lda + c0, lda, s0));
                 CRET(hipGetLastError());
                 const int right = panelEnd - c0 - wact;
                 if (right > 0 && m - c0 - wact > 0)
                     CRET(CGemmSub(m - c0 - wact, right, wact, a + static_cast<size_t>(c0) * lda + c0 + wact, lda, 0,
                                   a + static_cast<size_t>(c0 + wact) * lda + c0, lda,
-                                  a + static_cast<size_t>(c0 + wact) * lda + c0 + wact, lda, stream));
+                                  a + static_cast<size_t>(c0 + wact) * lda + c0 + wact, lda, s0));
             }
             c0 += wact;
         }
         if (pivoting)
+            hipLaunchKernelGGL(ccompact_moves_kernel, dim3(1), dim3(1024), 0, s0, m, j0, w.rowsrc, w.moved[k & 1]);
+        return hipGetLastError();
+    };
+    // Applies panel k to the columns [lo, hi) (interchanges, U12 solve, trailing product)
+    // and, when leftSwaps, its interchanges to the columns left of the panel.
+    auto update = [&](int k, int lo, int hi, bool leftSwaps, hipStream_t st) -> hipError_t
+    {
+        const int j0 = k * CNB, nb = std::min(CNB, kmax - j0), pe = j0 + nb;
+        const int cols = hi - lo, left = leftSwaps ? j0 : 0;
+        if (pivoting && cols + left > 0)
+            hipLaunchKernelGGL(couter_swap_kernel, dim3((cols + left + 3) / 4), dim3(256), 0, st, a, lda, 0, left, lo,
+                               hi, w.moved[k & 1]);
+        CRET(hipGetLastError());
+        if (cols <= 0) return hipSuccess;
+        CRET(CTrsmBlock(nb, cols, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, 0,
+                        a + static_cast<size_t>(lo) * lda + j0, lda, st));
+        if (m - pe <= 0) return hipSuccess;
+        ProfileScope prof(ProfileKernel::ComplexGemm, 8.0 * (m - pe) * cols * nb, st);
+        return CGemmSub(m - pe, cols, nb, a + static_cast<size_t>(j0) * lda + pe, lda, 0,
+                        a + static_cast<size_t>(lo) * lda + j0, lda, a + static_cast<size_t>(lo) * lda + pe, lda, st);
+    };
+
+    CRET(panel(0));
+    if (two) CRET(hipEventRecord(la->panel[0], s0));
+    for (int k = 0; k < npanel; ++k)
+    {
+        const int pe = std::min(kmax, (k + 1) * CNB);
+        const int nbNext = k + 1 < npanel ? std::min(CNB, kmax - pe) : 0;
+        // Bulk of the trailing update (s1): everything right of the next block column, and
+        // the interchanges of panel k left of it.
+        if (two) CRET(hipStreamWaitEvent(s1, la->panel[k & 1], 0));
+        CRET(update(k, pe + nbNext, n, true, s1));
+        if (two) CRET(hipEventRecord(la->bulk[k & 1], s1));
+        // Critical path (s0): the next block column, then its panel.
+        if (nbNext > 0)
         {
-            hipLaunchKernelGGL(ccompact_moves_kernel, dim3(1), dim3(1024), 0, stream, m, j0, w.rowsrc, w.moved);
-            const int outside = j0 + (n - panelEnd);
-            if (outside > 0)
-                hipLaunchKernelGGL(couter_swap_kernel, dim3((outside + 3) / 4), dim3(256), 0, stream, a, lda, j0, nb,
-                                   n, w.moved);
-            CRET(hipGetLastError());
-        }
-        const int right = n - panelEnd;
-        if (right > 0)
-        {
-            CRET(CTrsmBlock(nb, right, a + static_cast<size_t>(j0) * lda + j0, lda, true, true, 0,
-                            a + static_cast<size_t>(panelEnd) * lda + j0, lda, stream));
-            if (m - panelEnd > 0)
-            {
-                ProfileScope prof(ProfileKernel::ComplexGemm, 8.0 * (m - panelEnd) * right * nb, stream);
-                CRET(CGemmSub(m - panelEnd, right, nb, a + static_cast<size_t>(j0) * lda + panelEnd, lda, 0,
-                              a + static_cast<size_t>(panelEnd) * lda + j0, lda,
-                              a + static_cast<size_t>(panelEnd) * lda + panelEnd, lda, stream));
-            }
+            if (two && k > 0) CRET(hipStreamWaitEvent(s0, la->bulk[(k - 1) & 1], 0));
+            CRET(update(k, pe, pe + nbNext, false, s0));
+            CRET(panel(k + 1));
+            if (two) CRET(hipEventRecord(la->panel[(k + 1) & 1], s0));
         }
     }
-    if (m > kCLeafMultiRows) FoldWaitError(w.err, info, stream);
+    if (two) CRET(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
+    if (m > kCLeafMultiRows) FoldWaitError(w.err, info, s0);
     return hipGetLastError();
 }
 

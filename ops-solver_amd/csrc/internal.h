@@ -172,7 +172,10 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
 
 // ---- complex LU engine (complex_engine.hip) ----
 size_t ComplexGetrfWorkspaceBytes(int m, int n);
-hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream);
+// la (optional): look-ahead streams and events of the handle (the trailing updates then run
+// on la->side beside the next panel).
+hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, void* work, hipStream_t stream,
+                        const LookAhead* la = nullptr);
 size_t ComplexGetrsWorkspaceBytes(int n, int nrhs);
 hipError_t ComplexGetrs(int trans, int n, int nrhs, const float2* lu, int lda, const int* ipiv, float2* b, int ldb,
                         void* work, hipStream_t stream);
