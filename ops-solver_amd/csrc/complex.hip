@@ -267,7 +267,7 @@ __global__ __launch_bounds__(128) void cpanel_swap_trsm_kernel(float2* __restric
 // Slots, per column parity: the workgroups' keys (one word each: 16-bit tag, 32-bit pivot
 // score, 16-bit position rank; contiguous, so one poll of all keys touches few cache lines),
 // then their candidate rows (kCRowWords words each: 32-bit tag, 32-bit value).
-constexpr int kCRowWords = 32;  // 2 words per complex column (W <= 16)
+constexpr int kCRowWords = 64;  // 2 words per complex column (W <= 32)
 constexpr int kCMaxGroups = 128;
 constexpr int kCKeyWords = 2 * kCMaxGroups;  // both parities
 constexpr size_t kCSlotBytes =
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
             __syncthreads();
             if (lane < 2 * wact)
                 CStoreSlot(rows + g * kCRowWords + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
-            else if (lane == 2 * wact)
+            if (lane == 0)
                 __hip_atomic_store(keys + g,
                                    (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
                                        (top & 0xffffull),
@@ -859,7 +859,7 @@ hipError_t LaunchCLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int 
 
 int CLeafWidth(int mp)
 {
-    if (mp > kCLeafMultiRows) return 16;
+    if (mp > kCLeafMultiRows) return 32;
     if (mp <= 512) return 16;
     if (mp <= 4096) return 8;
     if (mp <= 8192) return 4;
@@ -872,7 +872,7 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
     const int mp = m - c0;
     if (mp > kCLeafMultiRows)
     {
-        constexpr int R = 2, W = 16;
+        constexpr int R = 2, W = 32;
         const int groups = (mp + 64 * R - 1) / (64 * R);
         if (groups > kCMaxGroups || wact > W || w.epoch == nullptr) return hipErrorInvalidValue;
         // Tags: W per leaf; key words carry 16 bits of them, which must not wrap within one
