@@ -309,14 +309,17 @@ __device__ inline unsigned long long CLoadSlot(const unsigned long long* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+template <bool LOCAL = false>
 __device__ inline void CStoreSlot(unsigned long long* p, unsigned tag, unsigned payload)
 {
     __hip_atomic_store(p, (static_cast<unsigned long long>(tag) << 32) | payload, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+                       LOCAL ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// FULL: wact == W (no column of the unrolled loop is skipped at run time).
-template <int R, int W, bool FULL>
+// FULL: wact == W (no column of the unrolled loop is skipped at run time). SPREAD: 8
+// workgroups per participant, only ids = 0 mod 8 take part, all on one XCD, and the exchange
+// words are stored at workgroup scope into its L2 (as leaf_multi_kernel's SPREAD).
+template <int R, int W, bool FULL, bool SPREAD = false>
 __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A, int lda, int m, int c0, int wact,
                                                          int pivoting, int newPanel, int* __restrict__ ipiv,
                                                          int* __restrict__ info, int* __restrict__ rowsrc,
@@ -330,7 +333,9 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
     __shared__ int s_bpos[W];
     __shared__ int s_count;
     const int lane = threadIdx.x;
-    const int P = gridDim.x, g = blockIdx.x;
+    if (SPREAD && (blockIdx.x & 7) != 0) return;  // not a participant
+    const int P = SPREAD ? static_cast<int>(gridDim.x >> 3) : static_cast<int>(gridDim.x);
+    const int g = SPREAD ? static_cast<int>(blockIdx.x >> 3) : static_cast<int>(blockIdx.x);
     const int mp = m - c0;
     float2* Ap = A + static_cast<size_t>(c0) * lda + c0;
     const int rbase = g * T * R;
@@ -394,12 +399,13 @@ __global__ __launch_bounds__(64) void cleaf_multi_kernel(float2* __restrict__ A,
                         }
             __syncthreads();
             if (lane < 2 * wact)
-                CStoreSlot(rows + g * kCRowWords + lane, tag, __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
+                CStoreSlot<SPREAD>(rows + g * kCRowWords + lane, tag,
+                                   __float_as_uint(s_row[lane < 2 * W ? lane : 0]));
             if (lane == 0)
                 __hip_atomic_store(keys + g,
                                    (static_cast<unsigned long long>(tag & 0xffffu) << 48) | ((top >> 32) << 16) |
                                        (top & 0xffffull),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                   __ATOMIC_RELAXED, SPREAD ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT);
             // the P keys: lane l reads workgroups l and l + 64
             unsigned long long key0 = 0ull, key1 = 0ull;
             bool ok = false;
@@ -888,6 +894,9 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
         int* err = w.err;
         void* args[] = {&A, &lda, &m, &c0, &wact, &pivoting, &newPanel, &ipiv, &info, &rowsrc, &leafmap, &slots,
                         &tagBase, &err, &spinLimit};
+        if (wact == W && LeafSpread())
+            return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W, true, true>),
+                                              dim3(8 * groups), dim3(64), args, 0, stream);
         const void* fn = wact == W ? reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W, true>)
                                    : reinterpret_cast<const void*>(&cleaf_multi_kernel<R, W, false>);
         return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(64), args, 0, stream);
@@ -1028,24 +1037,6 @@ hipError_t ComplexGetrf(int m, int n, float2* a, int lda, int* ipiv, int* info, 
 // ---------------------------------------------------------------------------------------
 namespace
 {
-__global__ __launch_bounds__(1024) void cipiv_to_perm_kernel(int n, const int* __restrict__ ipiv, int* __restrict__ perm)
-{
-    extern __shared__ int s_idx2[];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_idx2[i] = i;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < n; ++k)
-        {
-            int p = ipiv[k] - 1;
-            if (p < k || p >= n) p = k;
-            const int t = s_idx2[k];
-            s_idx2[k] = s_idx2[p];
-            s_idx2[p] = t;
-        }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = s_idx2[i];
-}
-
 template <bool GATHER>
 __global__ void cpermute_kernel(int n, int nrhs, const int* __restrict__ perm, const float2* __restrict__ src, int lds,
                                 float2* __restrict__ dst, int ldd)
@@ -1073,9 +1064,7 @@ hipError_t ComplexGetrs(int trans, int n, int nrhs, const float2* lu, int lda, c
     if (n > 16384) return hipErrorInvalidValue;
     float2* x = static_cast<float2*>(work);
     int* perm = reinterpret_cast<int*>(x + static_cast<size_t>(n) * nrhs);
-    if (ipiv)
-        hipLaunchKernelGGL(cipiv_to_perm_kernel, dim3(1), dim3(1024), static_cast<size_t>(n) * sizeof(int), stream, n,
-                           ipiv, perm);
+    if (ipiv) CRET(IpivToPerm(n, ipiv, perm, stream, true));
     const int* p = ipiv ? perm : nullptr;
     const dim3 grid((n + 255) / 256, nrhs);
     const int NBS = 128;

@@ -761,6 +761,8 @@ __global__ void xcc_probe_kernel()
     if (threadIdx.x == 0) g_xccOfGroup[blockIdx.x] = static_cast<int>(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu);
 }
 
+}  // namespace
+
 // Full-width tall leaves run spread over 8x the grid and exchange through one XCD's L2 when
 // the probe allowed it; OPS_SOLVER_LEAF_XCD=0 turns that off (read per launch).
 bool LeafSpread()
@@ -771,6 +773,9 @@ bool LeafSpread()
     if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) return false;
     return g_spreadOk[device].load(std::memory_order_relaxed) == 1;
 }
+
+namespace
+{
 
 hipError_t MultiLeaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd, int pivoting, int newPanel,
                      int* ipiv, int* info, const Work& w, hipStream_t stream)
@@ -1028,25 +1033,35 @@ hipError_t RealGetrf(int m, int n, float* a, int lda, int* ipiv, int* info, void
 namespace
 {
 // ipiv (LAPACK interchange sequence) -> permutation, perm[i] = source row of row i of P*B.
-// One workgroup; the sequential interchanges run on an LDS index array.
-__global__ __launch_bounds__(1024) void ipiv_to_perm_kernel(int n, const int* __restrict__ ipiv, int* __restrict__ perm)
+// With s_k the interchange of rows k and ipiv[k] - 1 (applied for k = 0, 1, ...), row i of
+// P*B comes from row s_0(s_1(... s_{n-1}(i))): one lane per position traces it backward
+// through the interchanges, ipiv staged in LDS (padded to a multiple of 4 with identities,
+// read four at a time as broadcasts). An out-of-range entry (and, with CLAMP_BELOW, one
+// pointing above its own row) is taken as no interchange.
+template <bool CLAMP_BELOW>
+__global__ __launch_bounds__(64) void ipiv_to_perm_kernel(int n, const int* __restrict__ ipiv, int* __restrict__ perm)
 {
-    extern __shared__ int s_idx[];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_idx[i] = i;
+    extern __shared__ int s_piv[];
+    const int n4 = (n + 3) & ~3;
+    for (int k = threadIdx.x; k < n4; k += 64)
+    {
+        int p = k < n ? ipiv[k] - 1 : k;
+        if (p < 0 || p >= n || (CLAMP_BELOW && p < k)) p = k;
+        s_piv[k] = p;
+    }
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < n; ++k)
-        {
-            const int p = ipiv[k] - 1;
-            if (p != k)
-            {
-                const int t = s_idx[k];
-                s_idx[k] = s_idx[p];
-                s_idx[p] = t;
-            }
-        }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = s_idx[i];
+    const int i = blockIdx.x * 64 + static_cast<int>(threadIdx.x);
+    int x = i;
+    const int4* q = reinterpret_cast<const int4*>(s_piv);
+    for (int k = n4 - 4; k >= 0; k -= 4)
+    {
+        const int4 p = q[k >> 2];
+        x = x == k + 3 ? p.w : (x == p.w ? k + 3 : x);
+        x = x == k + 2 ? p.z : (x == p.z ? k + 2 : x);
+        x = x == k + 1 ? p.y : (x == p.y ? k + 1 : x);
+        x = x == k ? p.x : (x == p.x ? k : x);
+    }
+    if (i < n) perm[i] = x;
 }
 
 // dst(:, c) = src(perm, c) (gather) or dst(perm, c) = src(:, c) (scatter).
@@ -1132,11 +1147,16 @@ hipError_t RealLuSolveInPlace(int trans, int n, int nrhs, const float* lu, int l
     return hipGetLastError();
 }
 
-hipError_t IpivToPerm(int n, const int* ipiv, int* perm, hipStream_t stream)
+hipError_t IpivToPerm(int n, const int* ipiv, int* perm, hipStream_t stream, bool clampBelow)
 {
     if (n > kMaxRows) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ipiv_to_perm_kernel, dim3(1), dim3(1024), static_cast<size_t>(n) * sizeof(int), stream, n, ipiv,
-                       perm);
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((n + 63) / 64);
+    const size_t lds = static_cast<size_t>((n + 3) & ~3) * sizeof(int);
+    if (clampBelow)
+        hipLaunchKernelGGL(ipiv_to_perm_kernel<true>, grid, dim3(64), lds, stream, n, ipiv, perm);
+    else
+        hipLaunchKernelGGL(ipiv_to_perm_kernel<false>, grid, dim3(64), lds, stream, n, ipiv, perm);
     return hipGetLastError();
 }
 

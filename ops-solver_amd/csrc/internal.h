@@ -95,6 +95,9 @@ void* NoAllocScratch(int device);
 // Once per device at handle creation: may the tall panel leaves exchange through one XCD's
 // L2 (getrf.hip)?
 void LeafSpreadProbe(int device);
+// May the full-width tall leaves of the current device run spread over 8x their grid and
+// exchange through one XCD's L2 (probe result, OPS_SOLVER_LEAF_XCD)?
+bool LeafSpread();
 
 // Writes value into count ints of a Device info array on stream.
 void FillInfo(int* info, int count, int value, hipStream_t stream);
@@ -171,6 +174,9 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
                      void* work, hipStream_t stream, int* info = nullptr);
 
 // ---- complex LU engine (complex_engine.hip) ----
+// ipiv (1-based interchange sequence) -> perm (perm[i] = source row of row i of P*B), one
+// launch; clampBelow also takes entries pointing above their own row as no interchange.
+hipError_t IpivToPerm(int n, const int* ipiv, int* perm, hipStream_t stream, bool clampBelow = false);
 size_t ComplexGetrfWorkspaceBytes(int m, int n);
 // la (optional): look-ahead streams and events of the handle (the trailing updates then run
 // on la->side beside the next panel).

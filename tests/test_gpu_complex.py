@@ -74,6 +74,29 @@ def test_cgetrf_parity(torch, handle, m, n, seed):
     assert O.lu_backward_error(a, lu, piv) / EPS32 <= 5
 
 
+@pytest.mark.parametrize("m,n,seed", [(5000, 200, 21), (2600, 2600, 22)])
+def test_ctall_leaf_exchange_paths_agree(torch, handle, m, n, seed, monkeypatch):
+    """The complex tall leaves' two exchange paths (spread over one XCD's L2, the default where
+    the device allows it, and the device-coherent path, OPS_SOLVER_LEAF_XCD=0) give identical
+    factors, with the oracle's pivots and the reference's residual bar."""
+    rng = np.random.default_rng(seed)
+    a = _crand(rng, m, n)
+    st, lu, piv, info = _cgetrf(torch, handle, a)
+    monkeypatch.setenv("OPS_SOLVER_LEAF_XCD", "0")
+    st0, lu0, piv0, info0 = _cgetrf(torch, handle, a)
+    monkeypatch.delenv("OPS_SOLVER_LEAF_XCD")
+    assert st == st0 == A.SUCCESS and info == info0 == 0
+    assert np.array_equal(lu.view(np.uint32), lu0.view(np.uint32))
+    assert np.array_equal(piv, piv0)
+    # the first kp pivots depend only on the first kp columns
+    kp = min(m, n, 192)
+    _, ref_piv, _, margin = O.cgetrf(np.ascontiguousarray(a[:, :kp]), margins=True)
+    bad = np.nonzero(margin < 1e-4)[0]
+    upto = int(bad[0]) if len(bad) else kp
+    np.testing.assert_array_equal(piv[:upto], ref_piv[:upto])
+    assert O.lu_backward_error(a, lu, piv) / EPS32 <= 5
+
+
 def test_cgetrf_tall_zero_column(torch, handle):
     """Tall panels (leaves spread over several workgroups): an exactly zero column gives
     info at that column, as the reference's first zero pivot."""
