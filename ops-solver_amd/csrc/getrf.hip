@@ -835,8 +835,8 @@ int LeafWidth(int mp, const Work& w)
 {
     if (UseTslu(w, mp)) return kTsluWidth;
     if (UseMultiLeaf(mp)) return kMultiW;
-    if (mp <= 512) return 32;
-    if (mp <= 1024) return 16;
+    if (mp <= 512) return 64;
+    if (mp <= 2048) return 32;
     if (mp <= 8192) return 8;
     return 4;
 }
@@ -853,10 +853,10 @@ hipError_t Leaf(float* A, int lda, int m, int j0, int c0, int wact, int panelEnd
     if (UseMultiLeaf(mp)) return MultiLeaf(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream);
 #define LEAF(T, R, W) \
     LaunchLeaf<T, R, W>(A, lda, m, j0, c0, wact, panelEnd, pivoting, newPanel, ipiv, info, w, stream)
-    if (mp <= 256) return LEAF(256, 1, 32);
-    if (mp <= 512) return LEAF(512, 1, 32);
-    if (mp <= 1024) return LEAF(512, 2, 16);
-    if (mp <= 2048) return LEAF(512, 4, 8);
+    if (mp <= 256) return LEAF(256, 1, 64);
+    if (mp <= 512) return LEAF(512, 1, 64);
+    if (mp <= 1024) return LEAF(512, 2, 32);
+    if (mp <= 2048) return LEAF(512, 4, 32);
     if (mp <= 4096) return LEAF(512, 8, 8);
     if (mp <= 8192) return LEAF(512, 16, 8);
     return LEAF(1024, 16, 4);
