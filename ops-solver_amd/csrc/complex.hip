@@ -178,7 +178,8 @@ __global__ __launch_bounds__(T) void cleaf_kernel(float2* __restrict__ A, int ld
                 const float2 l = below ? q : make_float2(0.0f, 0.0f);
                 a[i][j] = below ? q : a[i][j];
 #pragma unroll
-                for (int k = j + 1; k < W; ++k) a[i][k] = SubMul(a[i][k], l, s_prow[k]);
+                for (int k = 1; k < W; ++k)  // constant bounds: unrolled with the j loop
+                    if (k > j) a[i][k] = SubMul(a[i][k], l, s_prow[k]);
             }
             __syncthreads();
         }
@@ -866,7 +867,8 @@ hipError_t LaunchCLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int 
 int CLeafWidth(int mp)
 {
     if (mp > kCLeafMultiRows) return 32;
-    if (mp <= 512) return 16;
+    if (mp <= 512) return 32;
+    if (mp <= 2048) return 16;
     if (mp <= 4096) return 8;
     if (mp <= 8192) return 4;
     return 2;
@@ -902,10 +904,10 @@ hipError_t CLeaf(float2* A, int lda, int m, int j0, int c0, int wact, int pivoti
         return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(64), args, 0, stream);
     }
 #define CLEAF(T, R, W) LaunchCLeaf<T, R, W>(A, lda, m, j0, c0, wact, pivoting, newPanel, ipiv, info, w, stream)
-    if (mp <= 256) return CLEAF(256, 1, 16);
-    if (mp <= 512) return CLEAF(512, 1, 16);
-    if (mp <= 1024) return CLEAF(512, 2, 8);
-    if (mp <= 2048) return CLEAF(512, 4, 8);
+    if (mp <= 256) return CLEAF(256, 1, 32);
+    if (mp <= 512) return CLEAF(512, 1, 32);
+    if (mp <= 1024) return CLEAF(512, 2, 16);
+    if (mp <= 2048) return CLEAF(512, 4, 16);
     if (mp <= 4096) return CLEAF(512, 8, 8);
     if (mp <= 8192) return CLEAF(512, 16, 4);
     return CLEAF(512, 32, 2);
