@@ -1043,6 +1043,7 @@ __global__ __launch_bounds__(64) void ipiv_to_perm_kernel(int n, const int* __re
 {
     extern __shared__ int s_piv[];
     const int n4 = (n + 3) & ~3;
+#pragma unroll 16  // 16 loads per lane in flight
     for (int k = threadIdx.x; k < n4; k += 64)
     {
         int p = k < n ? ipiv[k] - 1 : k;
@@ -1053,6 +1054,7 @@ __global__ __launch_bounds__(64) void ipiv_to_perm_kernel(int n, const int* __re
     const int i = blockIdx.x * 64 + static_cast<int>(threadIdx.x);
     int x = i;
     const int4* q = reinterpret_cast<const int4*>(s_piv);
+#pragma unroll 4
     for (int k = n4 - 4; k >= 0; k -= 4)
     {
         const int4 p = q[k >> 2];
