@@ -1239,7 +1239,8 @@ hipError_t RealGetrs(int trans, int n, int nrhs, const float* lu, int lda, const
 
 // Records the XCD of every workgroup of a 1024-workgroup cooperative launch and allows the
 // spread tall leaves on this device when those with ids = 0 mod 8 share one XCD. Runs once
-// per device (the device current), synchronously on the null stream.
+// per device (the device current), synchronously on the null stream; two threads probing
+// one device at once both store the same result.
 void LeafSpreadProbe(int device)
 {
     if (device < 0 || device >= kMaxDevices || g_spreadOk[device].load() != 0) return;
@@ -1249,7 +1250,7 @@ void LeafSpreadProbe(int device)
                                    nullptr) == hipSuccess &&
         hipStreamSynchronize(nullptr) == hipSuccess)
     {
-        static int h[G];
+        int h[G];
         if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_xccOfGroup), sizeof(h)) == hipSuccess)
         {
             result = 1;
