@@ -58,6 +58,8 @@ struct Work
 {
     int* rowsrc;
     int* moved[2];
+    int* movedAll;  // every panel's moved-row list (deferred interchanges), kMovedInts each
+    int* composed;  // m ints: composed row map of the deferred interchanges
     int* perm;
     unsigned long long* slots;
     int* err;
@@ -67,7 +69,9 @@ struct Work
     void* tslu;            // scratch of the tournament leaves
     bool tournament;       // tall leaves pivot by tournament (TsluLeaf)
 };
-inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 2 + 2 * kMovedInts; }
+// Panels of one factorization are at least 128 columns wide.
+inline size_t MaxPanels(int m) { return static_cast<size_t>(m) / 128 + 2; }
+inline size_t WorkInts(int m) { return static_cast<size_t>(m) * 3 + (2 + MaxPanels(m)) * kMovedInts; }
 inline Work Layout(void* base, int m)
 {
     int* p = static_cast<int*>(base);
@@ -76,6 +80,8 @@ inline Work Layout(void* base, int m)
     w.moved[0] = p + m;
     w.moved[1] = w.moved[0] + kMovedInts;
     w.perm = w.moved[1] + kMovedInts;
+    w.composed = w.perm + m;
+    w.movedAll = w.composed + m;
     uintptr_t q = reinterpret_cast<uintptr_t>(p + WorkInts(m));
     q = (q + 7) & ~static_cast<uintptr_t>(7);
     w.slots = reinterpret_cast<unsigned long long*>(q);
@@ -705,6 +711,35 @@ __global__ __launch_bounds__(256) void outer_swap_kernel(float* __restrict__ A, 
         if (d[q] >= 0) col[d[q]] = v[q];
 }
 
+// Deferred interchanges of the columns left of later panels. composed[r] <- mu(composed[r])
+// for r in [lo, m), with mu the row map of one panel's moved-row list (dst -> src; every
+// dst and src of that list is >= lo), built in LDS. One workgroup.
+__global__ __launch_bounds__(1024) void compose_moves_kernel(int m, int lo, const int* __restrict__ moved,
+                                                             int* __restrict__ composed)
+{
+    extern __shared__ int s_map[];  // m - lo entries
+    const int len = m - lo;
+    for (int r = threadIdx.x; r < len; r += blockDim.x) s_map[r] = lo + r;
+    __syncthreads();
+    const int cnt = moved[0];
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) s_map[moved[1 + 2 * e] - lo] = moved[2 + 2 * e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < len; r += blockDim.x) composed[lo + r] = s_map[composed[lo + r] - lo];
+}
+
+// Column c0 + blockIdx.x, rows [lo, m): row r takes the value of row composed[r], gathered
+// through LDS (the whole column part first, then written back in order).
+__global__ __launch_bounds__(256) void gather_rows_kernel(float* __restrict__ A, int lda, int c0, int lo, int m,
+                                                          const int* __restrict__ composed)
+{
+    extern __shared__ float s_col[];  // m - lo values
+    float* col = A + static_cast<size_t>(c0 + blockIdx.x) * lda;
+    const int len = m - lo;
+    for (int r = threadIdx.x; r < len; r += blockDim.x) s_col[r] = col[composed[lo + r]];
+    __syncthreads();
+    for (int r = threadIdx.x; r < len; r += blockDim.x) col[lo + r] = s_col[r];
+}
+
 __global__ void iota_kernel(int* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -945,6 +980,15 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
     if (permOut) hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, stream, permOut, m);
     const int NB = OuterBlock(m, n, trailing);
     const int npanel = (kmax + NB - 1) / NB;
+    // Deferred interchanges (default; OPS_SOLVER_DEFER_SWAPS=0 turns them off): the columns
+    // left of a panel receive its interchanges not panel by panel but once at the end,
+    // through the composition of all later panels' row maps (every panel keeps its moved-row
+    // list for that): about n^2/2 gathers instead of ~2 n^2 scattered reads and writes,
+    // which in column-major storage cost a cache line each and compete with the panel.
+    const char* dz = std::getenv("OPS_SOLVER_DEFER_SWAPS");
+    const bool defer = pivoting && npanel > 1 && (dz == nullptr || std::atoi(dz) != 0) &&
+                       static_cast<size_t>(npanel) <= MaxPanels(m);
+    auto moved = [&](int k) { return defer ? w.movedAll + static_cast<size_t>(k) * kMovedInts : w.moved[k & 1]; };
 
     // Look-ahead: the caller's stream s0 carries the critical path (the next block column's
     // update and its panel); the side stream s1 carries the rest of each trailing update.
@@ -964,7 +1008,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         const int j0 = k * NB, nb = std::min(NB, kmax - j0);
         RET_IF(PanelRec(a, lda, m, j0, j0 + nb, j0, nb, pivoting, ipiv, info, w, s0));
         if (pivoting)
-            hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, s0, m, j0, w.rowsrc, w.moved[k & 1],
+            hipLaunchKernelGGL(compact_moves_kernel, dim3(1), dim3(1024), 0, s0, m, j0, w.rowsrc, moved(k),
                                permOut);
         RET_IF(hipGetLastError());
         // The U12 solves of both streams use the block inverses of this panel's L11.
@@ -978,7 +1022,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         const int cols = hi - lo, left = leftSwaps ? j0 : 0;
         if (pivoting && cols + left > 0)
             hipLaunchKernelGGL(outer_swap_kernel, dim3((cols + left + 3) / 4), dim3(256), 0, st, a, lda, 0, left, lo,
-                               hi, w.moved[k & 1]);
+                               hi, moved(k));
         RET_IF(hipGetLastError());
         if (cols <= 0) return hipSuccess;
         RET_IF(TrsmLowerUnitApply(nb, cols, a + static_cast<size_t>(j0) * lda + j0, lda, w.linvPanel[k & 1],
@@ -1000,7 +1044,7 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         const int nbNext = k + 1 < npanel ? std::min(NB, kmax - pe) : 0;
         // Bulk of the trailing update (s1): everything right of the next block column.
         if (two) RET_IF(hipStreamWaitEvent(s1, la->panel[k & 1], 0));
-        RET_IF(update(k, pe + nbNext, n, true, true, s1));
+        RET_IF(update(k, pe + nbNext, n, !defer, true, s1));
         if (two) RET_IF(hipEventRecord(la->bulk[k & 1], s1));
         // Critical path (s0): the next block column, then its panel.
         if (nbNext > 0)
@@ -1012,6 +1056,21 @@ hipError_t RealGetrfPerm(int m, int n, float* a, int lda, int* ipiv, int* info, 
         }
     }
     if (two) RET_IF(hipStreamWaitEvent(s0, la->bulk[(npanel - 1) & 1], 0));
+    if (defer)
+    {
+        // Backward over the panels: composed = mu_{p+1} o ... o mu_last, then panel p's
+        // columns gather their rows [(p+1) NB, m) through it.
+        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, s0, w.composed, m);
+        for (int p = npanel - 2; p >= 0; --p)
+        {
+            const int lo = (p + 1) * NB, nb = std::min(NB, kmax - p * NB);
+            if (lo >= m) continue;
+            const size_t lds = static_cast<size_t>(m - lo) * sizeof(int);
+            hipLaunchKernelGGL(compose_moves_kernel, dim3(1), dim3(1024), lds, s0, m, lo, moved(p + 1), w.composed);
+            hipLaunchKernelGGL(gather_rows_kernel, dim3(nb), dim3(256), lds, s0, a, lda, p * NB, lo, m, w.composed);
+        }
+        RET_IF(hipGetLastError());
+    }
     FoldWaitError(w.err, info, s0);
     if (split)
     {

@@ -202,6 +202,23 @@ def test_determinism(torch, handle):
         assert np.array_equal(piv, outs[0][1]) and info == outs[0][2]
 
 
+@pytest.mark.parametrize("m,n,seed", [(3000, 3000, 7), (4500, 1300, 8), (1300, 2000, 9)])
+def test_deferred_interchanges_agree(torch, handle, m, n, seed, monkeypatch):
+    """The interchanges of the columns left of each panel, applied once at the end through
+    the composed row map (the default) or panel by panel (OPS_SOLVER_DEFER_SWAPS=0), give
+    identical factors; unshifted N(0,1) entries pivot on nearly every column."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((m, n)).astype(np.float32)
+    st, lu, piv, info, _ = _sgetrf(torch, handle, a)
+    monkeypatch.setenv("OPS_SOLVER_DEFER_SWAPS", "0")
+    st0, lu0, piv0, info0, _ = _sgetrf(torch, handle, a)
+    monkeypatch.delenv("OPS_SOLVER_DEFER_SWAPS")
+    assert st == st0 == A.SUCCESS and info == info0 == 0
+    assert np.array_equal(piv, piv0)
+    assert np.array_equal(lu.view(np.uint32), lu0.view(np.uint32))
+    assert O.lu_backward_error(a, lu, piv) <= 10 * max(m, n) * EPS32
+
+
 @pytest.mark.parametrize("m,n,seed", [(6000, 200, 5), (3000, 3000, 6)])
 def test_tall_leaf_exchange_paths_agree(torch, handle, m, n, seed, monkeypatch):
     """The tall leaves' two exchange paths (spread over one XCD's L2, the default where the
